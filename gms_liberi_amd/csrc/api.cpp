@@ -1,0 +1,570 @@
+// C ABI of the MI355X Fock build: handle lifecycle, setup, build.
+//
+// Host side of the drop-in for src/interface/liberi_interface.F90 (handle
+// API), src/liberi_shell_pair.F90 (setup) and src/liberi_driver.F90
+// (compute_integrals) of gms_libERI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/liberi_hip.h"
+#include "liberi_internal.hpp"
+
+namespace liberi {
+
+const ClassDef kClasses[kNumClasses] = {
+    {"int0000", P_SS, P_SS, CMP_GE, PS_SELF},     {"int0001", P_SS, P_SP, CMP_GT, PS_CROSS},
+    {"int0011", P_SS, P_PP, CMP_GT, PS_CROSS},    {"int0111", P_SP, P_PP, CMP_GT, PS_CROSS},
+    {"int0101", P_SP, P_SP, CMP_GE, PS_SELF},     {"int1111", P_PP, P_PP, CMP_GE, PS_SELF},
+    {"int0002", P_SS, P_SD, CMP_GT, PS_CROSS},    {"int0012", P_SS, P_PD, CMP_GT, PS_CROSS},
+    {"int0102", P_SP, P_SD, CMP_GT, PS_CROSS},    {"int0112", P_SP, P_PD, CMP_GT, PS_CROSS},
+    {"int0022", P_SS, P_DD, CMP_GT, PS_CROSS},    {"int0122", P_SP, P_DD, CMP_GT, PS_CROSS},
+    {"int1102", P_PP, P_SD, CMP_GT, PS_CROSS},    {"int0202", P_SD, P_SD, CMP_GT, PS_SELF},
+    {"int2211", P_DD, P_PP, CMP_GT, PS_SELF},     {"int2121", P_PD, P_PD, CMP_GT, PS_SELF},
+    {"int2120", P_PD, P_SD, CMP_GT, PS_SELF},     {"int2111", P_PD, P_PP, CMP_GT, PS_SELF},
+    {"int2220", P_DD, P_SD, CMP_GT, PS_SELF},     {"int2221", P_DD, P_PD, CMP_GT, PS_SELF},
+    {"int2222gen", P_DD, P_DD, CMP_NONE, PS_SELF}, {"int3333", P_FF, P_FF, CMP_NONE, PS_SELF},
+    {"int0030", P_SS, P_SF, CMP_GT, PS_SELF},     {"int0031", P_SS, P_PF, CMP_GT, PS_SELF},
+    {"int0032", P_SS, P_DF, CMP_GT, PS_SELF},     {"int1030", P_SP, P_SF, CMP_GT, PS_SELF},
+    {"int3330", P_FF, P_SF, CMP_GT, PS_SELF},     {"int3011", P_SF, P_PP, CMP_GT, PS_SELF},
+    {"int3020", P_SF, P_SD, CMP_GT, PS_SELF},     {"int2130", P_PD, P_SF, CMP_GT, PS_SELF},
+    {"int2230", P_DD, P_SF, CMP_GT, PS_SELF},     {"int3030", P_SF, P_SF, CMP_GT, PS_SELF},
+    {"int3110", P_PF, P_SP, CMP_GT, PS_SELF},     {"int3111", P_PF, P_PP, CMP_GT, PS_SELF},
+    {"int3120", P_PF, P_SD, CMP_GT, PS_SELF},     {"int3121", P_PF, P_PD, CMP_GT, PS_SELF},
+    {"int2231", P_DD, P_PF, CMP_GT, PS_SELF},     {"int3130", P_PF, P_SF, CMP_GT, PS_SELF},
+    {"int3131", P_PF, P_PF, CMP_GT, PS_SELF},     {"int3210", P_DF, P_SP, CMP_GT, PS_SELF},
+    {"int3331", P_FF, P_PF, CMP_GT, PS_SELF},     {"int3211", P_DF, P_PP, CMP_GT, PS_SELF},
+    {"int3220", P_DF, P_SD, CMP_GT, PS_SELF},     {"int3221", P_DF, P_PD, CMP_GT, PS_SELF},
+    {"int3222", P_DF, P_DD, CMP_GT, PS_SELF},     {"int3230", P_DF, P_SF, CMP_GT, PS_SELF},
+    {"int3231", P_DF, P_PF, CMP_GT, PS_SELF},     {"int3232", P_DF, P_DF, CMP_GT, PS_SELF},
+    {"int3300", P_FF, P_SS, CMP_GT, PS_SELF},     {"int3310", P_FF, P_SP, CMP_GT, PS_SELF},
+    {"int3311", P_FF, P_PP, CMP_GT, PS_SELF},     {"int3320", P_FF, P_SD, CMP_GT, PS_SELF},
+    {"int3321", P_FF, P_PD, CMP_GT, PS_SELF},     {"int3322", P_FF, P_DD, CMP_GT, PS_SELF},
+    {"int3332", P_FF, P_DF, CMP_GT, PS_SELF},
+};
+
+}  // namespace liberi
+
+using namespace liberi;
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_last_error;
+
+static int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                                   \
+  do {                                                                                                  \
+    hipError_t e_ = (expr);                                                                             \
+    if (e_ != hipSuccess)                                                                               \
+      return fail(e_ == hipErrorOutOfMemory ? LIBERI_ENOMEM : LIBERI_EDEVICE,                           \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                                   \
+  } while (0)
+
+// --------------------------------------------------------------- handle
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+struct ClassState {
+  bool active = false;
+  int64_t nquart = 0, nprimq = 0;
+  int32_t nseg = 0;
+  DevBuf ket_list, seg_start, seg_bra, seg_ket0;
+  ClassDev dev{};
+};
+
+struct liberi_ctx {
+  // options
+  double schwarz_cut = kRefCutoff, value_cut = kRefCutoff, fp32_tau = 0.0, exact_large_t = 0.0;
+  // basis
+  bool initialized = false;
+  int32_t nsh = 0, natoms = 0, num_bas = 0, mxgtot = 0, rank = 0, nprocs = 1;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<DevBuf> owned;
+  DevBuf pairs, prims, dens, fock, scratch;
+  int64_t scratch_per_wg = 0;
+  ClassState cls[kNumClasses];
+  liberi_stats stats{};
+};
+
+static int dev_alloc(liberi_ctx *h, DevBuf &b, size_t bytes) {
+  b.bytes = bytes;
+  if (bytes == 0) {
+    b.p = nullptr;
+    return LIBERI_OK;
+  }
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess)
+    return fail(e == hipErrorOutOfMemory ? LIBERI_ENOMEM : LIBERI_EDEVICE,
+                "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+  h->stats.device_bytes += (int64_t)bytes;
+  return LIBERI_OK;
+}
+
+template <class T>
+static int dev_upload(liberi_ctx *h, DevBuf &b, const std::vector<T> &v) {
+  int rc = dev_alloc(h, b, v.size() * sizeof(T));
+  if (rc) return rc;
+  if (!v.empty()) HIP_TRY(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return LIBERI_OK;
+}
+
+static void dev_free(DevBuf &b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+static void release_device(liberi_ctx *h) {
+  for (auto &c : h->cls) {
+    dev_free(c.ket_list);
+    dev_free(c.seg_start);
+    dev_free(c.seg_bra);
+    dev_free(c.seg_ket0);
+    c = ClassState{};
+  }
+  dev_free(h->pairs);
+  dev_free(h->prims);
+  dev_free(h->dens);
+  dev_free(h->fock);
+  dev_free(h->scratch);
+  h->stats = liberi_stats{};
+  h->initialized = false;
+}
+
+extern "C" {
+
+const char *liberi_c_last_error(void) { return g_last_error.c_str(); }
+
+const char *liberi_c_class_name(int32_t c) { return (c >= 0 && c < kNumClasses) ? kClasses[c].name : ""; }
+
+int liberi_c_create(liberi_ctx **ctx) {
+  if (!ctx) return fail(LIBERI_EARG, "liberi_c_create: NULL handle pointer");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(LIBERI_EDEVICE, "liberi_c_create: no HIP device visible");
+  auto *h = new (std::nothrow) liberi_ctx();
+  if (!h) return fail(LIBERI_ENOMEM, "liberi_c_create: out of host memory");
+  HIP_TRY(hipGetDevice(&h->device));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, h->device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    delete h;
+    return fail(LIBERI_EDEVICE, std::string("liberi_c_create: device is ") + prop.gcnArchName +
+                                    ", this library is built for gfx950 (MI355X)");
+  }
+  HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&h->ev0));
+  HIP_TRY(hipEventCreate(&h->ev1));
+  *ctx = h;
+  return LIBERI_OK;
+}
+
+int liberi_c_cleanup(liberi_ctx *h) {
+  if (!h) return fail(LIBERI_EARG, "liberi_c_cleanup: NULL handle");
+  if (!h->initialized) return LIBERI_OK;  // src/interface/liberi_interface.F90:235
+  (void)hipStreamSynchronize(h->stream);
+  release_device(h);
+  return LIBERI_OK;
+}
+
+int liberi_c_destroy(liberi_ctx **ctx) {
+  if (!ctx) return fail(LIBERI_EARG, "liberi_c_destroy: NULL handle pointer");
+  liberi_ctx *h = *ctx;
+  if (!h) return LIBERI_OK;
+  liberi_c_cleanup(h);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  *ctx = nullptr;
+  return LIBERI_OK;
+}
+
+int liberi_c_set_option(liberi_ctx *h, int32_t opt, double v) {
+  if (!h) return fail(LIBERI_EARG, "liberi_c_set_option: NULL handle");
+  if (!(v >= 0.0) || !std::isfinite(v)) return fail(LIBERI_EARG, "liberi_c_set_option: value must be finite, >= 0");
+  switch (opt) {
+    case LIBERI_OPT_SCHWARZ_CUTOFF: h->schwarz_cut = v; break;
+    case LIBERI_OPT_VALUE_CUTOFF: h->value_cut = v; break;
+    case LIBERI_OPT_FP32_TAU: h->fp32_tau = v; break;
+    case LIBERI_OPT_EXACT_LARGE_T: h->exact_large_t = v != 0.0; break;
+    default: return fail(LIBERI_EARG, "liberi_c_set_option: unknown option " + std::to_string(opt));
+  }
+  return LIBERI_OK;
+}
+
+int liberi_c_get_option(liberi_ctx *h, int32_t opt, double *v) {
+  if (!h || !v) return fail(LIBERI_EARG, "liberi_c_get_option: NULL argument");
+  switch (opt) {
+    case LIBERI_OPT_SCHWARZ_CUTOFF: *v = h->schwarz_cut; break;
+    case LIBERI_OPT_VALUE_CUTOFF: *v = h->value_cut; break;
+    case LIBERI_OPT_FP32_TAU: *v = h->fp32_tau; break;
+    case LIBERI_OPT_EXACT_LARGE_T: *v = h->exact_large_t; break;
+    default: return fail(LIBERI_EARG, "liberi_c_get_option: unknown option " + std::to_string(opt));
+  }
+  return LIBERI_OK;
+}
+
+int liberi_c_get_stats(liberi_ctx *h, liberi_stats *out) {
+  if (!h || !out) return fail(LIBERI_EARG, "liberi_c_get_stats: NULL argument");
+  *out = h->stats;
+  return LIBERI_OK;
+}
+
+// ------------------------------------------------------------------ setup
+// Where the reference stops using its tabulated Boys function / Rys roots.
+// Rotated-axis and ERIC kernels (classes 0-13 and 22-25) take the asymptotic
+// Boys chain F_m = F_{m-1} (2m-1)/2T for T > t_max = 25
+// (src/liberi_boys.F90:22, src/rhf/rot_axis/int1111.F90:389-396,
+// src/rhf/eric/int0030_ericgen.F90:300-311); the Rys kernels take the
+// asymptotic roots for X >= 37, 43, 49, 55, 60, 65 with 2..7 roots
+// (e.g. src/rhf/rot_axis/int2222_rysgen.F90:642).  Switching the Rys rule at
+// the same points reproduces those values.
+static double reference_switch(int c) {
+  if (c <= 13 || (c >= 22 && c <= 25)) return std::nextafter(25.0, INFINITY);  // table iff T <= 25
+  static const double thr[8] = {0, 0, 37.0, 43.0, 49.0, 55.0, 60.0, 65.0};       // table iff X < thr
+  return thr[class_nroots(c)];
+}
+
+struct HostShell {
+  int l, k, bf0;
+  double c[3];
+  const double *e, *w;
+};
+
+int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, int32_t mxgtot,
+                   const int32_t *ang_mom, const int32_t *contr_num, const int32_t *sh_loc, const int32_t *atom_num,
+                   const int32_t *atom_loc, const int32_t *start_bas, const int32_t *end_bas,
+                   const double *exponents, const double *cs, const double *cp, const double *cdc, const double *cf,
+                   const double *coords, const double *schwrz, int32_t my_rank, int32_t num_procs) {
+  (void)start_bas;  // stored but never read by the reference's hot path either
+  (void)end_bas;
+  if (!h) return fail(LIBERI_EARG, "liberi_c_setup: NULL handle");
+  if (nsh <= 0 || natoms <= 0 || num_bas <= 0 || mxgtot <= 0)
+    return fail(LIBERI_EARG, "liberi_c_setup: nsh, natoms, num_bas, mxgtot must be positive");
+  if (!ang_mom || !contr_num || !sh_loc || !atom_num || !atom_loc || !exponents || !cs || !cp || !cdc || !cf ||
+      !coords || !schwrz)
+    return fail(LIBERI_EARG, "liberi_c_setup: NULL array");
+  if (num_procs < 1 || my_rank < 0 || my_rank >= num_procs)
+    return fail(LIBERI_EARG, "liberi_c_setup: need 0 <= my_rank < num_procs");
+  if (h->initialized) liberi_c_cleanup(h);  // src/interface/liberi_interface.F90:133-136
+
+  // ---- shells (src/liberi_shell_pair.F90:108-135)
+  std::vector<HostShell> sh(nsh);
+  const int64_t coef_len = (int64_t)mxgtot * nsh;
+  for (int s = 0; s < nsh; ++s) {
+    const int l = ang_mom[s] - 1, k = contr_num[s], loc = sh_loc[s] - 1, a = atom_num[s] - 1;
+    if (l < 0 || l > 3) return fail(LIBERI_EARG, "liberi_c_setup: ang_mom must be 1..4 (shell " + std::to_string(s + 1) + ")");
+    if (k < 1 || loc < 0 || loc + k > coef_len)
+      return fail(LIBERI_EARG, "liberi_c_setup: contr_num/sh_loc out of range (shell " + std::to_string(s + 1) + ")");
+    if (a < 0 || a >= natoms) return fail(LIBERI_EARG, "liberi_c_setup: atom_num out of range");
+    const int bf0 = atom_loc[s] - 1;
+    if (bf0 < 0 || bf0 + (l + 1) * (l + 2) / 2 > num_bas) return fail(LIBERI_EARG, "liberi_c_setup: atom_loc out of range");
+    const double *w = l == 0 ? cs : l == 1 ? cp : l == 2 ? cdc : cf;
+    for (int i = 0; i < k; ++i)
+      if (!(exponents[loc + i] > 0.0) || !std::isfinite(exponents[loc + i]) || !std::isfinite(w[loc + i]))
+        return fail(LIBERI_EARG, "liberi_c_setup: exponents must be finite and > 0, coefficients finite (shell " +
+                                     std::to_string(s + 1) + ")");
+    sh[s] = HostShell{l, k, bf0, {coords[3 * a], coords[3 * a + 1], coords[3 * a + 2]}, exponents + loc, w + loc};
+  }
+  h->nsh = nsh;
+  h->natoms = natoms;
+  h->num_bas = num_bas;
+  h->mxgtot = mxgtot;
+  h->rank = my_rank;
+  h->nprocs = num_procs;
+
+  // ---- shell lists per L, ascending index (src/liberi_shell_pair.F90:137-344)
+  std::vector<int> lists[4];
+  for (int s = 0; s < nsh; ++s) lists[sh[s].l].push_back(s);
+
+  // ---- pair classes: records, Schwarz values (compute_*_pairs, :348-1682)
+  std::vector<PairRec> pairs;
+  std::vector<PrimRec> prims;
+  std::vector<double> pair_x;
+  std::vector<std::vector<int32_t>> class_pairs(kNumPairClasses);
+  auto add_pair = [&](int pc, int sA, int sB) {
+    const HostShell &A = sh[sA], &B = sh[sB];
+    PairRec r{};
+    r.prim_off = (int32_t)prims.size();
+    r.nprim = A.k * B.k;
+    r.bfA = A.bf0;
+    r.bfB = B.bf0;
+    r.ABx = A.c[0] - B.c[0];
+    r.ABy = A.c[1] - B.c[1];
+    r.ABz = A.c[2] - B.c[2];
+    r.sameAB = sA == sB;
+    r.shA = sA;
+    const double ab2 = r.ABx * r.ABx + r.ABy * r.ABy + r.ABz * r.ABz;
+    for (int i = 0; i < A.k; ++i)
+      for (int j = 0; j < B.k; ++j) {
+        const double a = A.e[i], b = B.e[j], p = a + b;
+        PrimRec pr;
+        pr.p = p;
+        pr.Px = (a * A.c[0] + b * B.c[0]) / p;
+        pr.Py = (a * A.c[1] + b * B.c[1]) / p;
+        pr.Pz = (a * A.c[2] + b * B.c[2]) / p;
+        pr.PAx = pr.Px - A.c[0];
+        pr.PAy = pr.Py - A.c[1];
+        pr.PAz = pr.Pz - A.c[2];
+        pr.K = A.w[i] * B.w[j] * std::exp(-a * b / p * ab2) / p;
+        prims.push_back(pr);
+      }
+    const int hi = std::max(sA, sB), lo = std::min(sA, sB);
+    pair_x.push_back(schwrz[(int64_t)hi * (hi + 1) / 2 + lo]);
+    class_pairs[pc].push_back((int32_t)pairs.size());
+    pairs.push_back(r);
+  };
+  for (int pc = 0; pc < kNumPairClasses; ++pc) {
+    const int l1 = kPairL[pc][0], l2 = kPairL[pc][1];
+    if (l1 == l2) {
+      for (size_t ii = 0; ii < lists[l1].size(); ++ii)
+        for (size_t jj = 0; jj <= ii; ++jj) add_pair(pc, lists[l1][ii], lists[l1][jj]);
+    } else {
+      for (int x : lists[l1])
+        for (int y : lists[l2]) add_pair(pc, y, x);  // A = the higher-L shell
+    }
+  }
+  if (prims.size() > (size_t)INT32_MAX || pairs.size() > (size_t)INT32_MAX)
+    return fail(LIBERI_EARG, "liberi_c_setup: basis too large for 32-bit pair indices");
+
+  int rc;
+  if ((rc = dev_upload(h, h->pairs, pairs))) return rc;
+  if ((rc = dev_upload(h, h->prims, prims))) return rc;
+  const int64_t ntri = (int64_t)num_bas * (num_bas + 1) / 2;
+  if ((rc = dev_alloc(h, h->dens, ntri * sizeof(double)))) return rc;
+  if ((rc = dev_alloc(h, h->fock, ntri * sizeof(double)))) return rc;
+
+  // ---- per class: pair pre-screen (host), quartet segments (device)
+  std::vector<DevBuf> tmp;
+  auto cleanup_tmp = [&]() {
+    for (auto &b : tmp) dev_free(b);
+    tmp.clear();
+  };
+  int64_t max_scratch = 0;
+  for (int c = 0; c < kNumClasses; ++c) {
+    const ClassDef &def = kClasses[c];
+    ClassState &cs_ = h->cls[c];
+    const auto &lb = class_pairs[def.bra], &lk = class_pairs[def.ket];
+    int lmax = std::max(std::max(kPairL[def.bra][0], kPairL[def.bra][1]), std::max(kPairL[def.ket][0], kPairL[def.ket][1]));
+    // the driver calls the class only if its highest shell type exists
+    // (src/liberi_driver.F90:69,72,80,105); an empty pair list has no quartets
+    if (lists[lmax].empty() || lb.empty() || lk.empty()) continue;
+    auto maxval = [&](const std::vector<int32_t> &l) {
+      double m = -INFINITY;
+      for (int32_t p : l) m = std::max(m, pair_x[p]);
+      return m;
+    };
+    const double mb = maxval(lb), mk = maxval(lk);
+    const double cut = h->schwarz_cut;
+    const double sb = def.ps == PS_CROSS ? cut / mk : cut / mb;
+    const double sk = def.ps == PS_CROSS ? cut / mb : cut / mk;
+    const bool diag = def.bra == def.ket;
+    auto keep = [&](const std::vector<int32_t> &l, double thr) {
+      std::vector<int32_t> out;
+      for (int32_t p : l)
+        if (pair_x[p] >= thr) out.push_back(p);
+      // sort by (primitive count, Schwarz value descending, pair index)
+      std::sort(out.begin(), out.end(), [&](int32_t a, int32_t b) {
+        if (pairs[a].nprim != pairs[b].nprim) return pairs[a].nprim < pairs[b].nprim;
+        if (pair_x[a] != pair_x[b]) return pair_x[a] > pair_x[b];
+        return a < b;
+      });
+      return out;
+    };
+    std::vector<int32_t> kb = keep(lb, sb);
+    std::vector<int32_t> kk = diag ? kb : keep(lk, sk);
+    if (kb.empty() || kk.empty()) continue;
+    // ket groups (equal primitive count)
+    std::vector<int32_t> gbeg, gend, kgroup(kk.size());
+    for (size_t j = 0; j < kk.size(); ++j) {
+      if (j == 0 || pairs[kk[j]].nprim != pairs[kk[j - 1]].nprim) {
+        if (j) gend.push_back((int32_t)j);
+        gbeg.push_back((int32_t)j);
+      }
+      kgroup[j] = (int32_t)gbeg.size() - 1;
+    }
+    gend.push_back((int32_t)kk.size());
+    const int ng = (int)gbeg.size();
+    std::vector<double> bx(kb.size()), kx(kk.size());
+    std::vector<int32_t> bgroup(kb.size()), bpos(kb.size());
+    for (size_t i = 0; i < kb.size(); ++i) {
+      bx[i] = pair_x[kb[i]];
+      bgroup[i] = diag ? kgroup[i] : 0;
+      bpos[i] = (int32_t)i;
+    }
+    for (size_t j = 0; j < kk.size(); ++j) kx[j] = pair_x[kk[j]];
+    DevBuf d_bx, d_bg, d_bp, d_kx, d_gb, d_ge, d_len;
+    if ((rc = dev_upload(h, d_bx, bx)) || (rc = dev_upload(h, d_bg, bgroup)) || (rc = dev_upload(h, d_bp, bpos)) ||
+        (rc = dev_upload(h, d_kx, kx)) || (rc = dev_upload(h, d_gb, gbeg)) || (rc = dev_upload(h, d_ge, gend))) {
+      cleanup_tmp();
+      return rc;
+    }
+    tmp = {d_bx, d_bg, d_bp, d_kx, d_gb, d_ge};
+    const int64_t nsegc = (int64_t)kb.size() * ng;
+    if ((rc = dev_alloc(h, d_len, nsegc * sizeof(int64_t)))) {
+      cleanup_tmp();
+      return rc;
+    }
+    tmp.push_back(d_len);
+    int e = launch_segment_lengths((const double *)d_bx.p, (const int32_t *)d_bg.p, (const int32_t *)d_bp.p,
+                                   (int32_t)kb.size(), (const double *)d_kx.p, (const int32_t *)d_gb.p,
+                                   (const int32_t *)d_ge.p, ng, diag, def.cmp, cut, (int64_t *)d_len.p, h->stream);
+    std::vector<int64_t> len(nsegc);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(len.data(), d_len.p, nsegc * sizeof(int64_t), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    for (auto &b : tmp) h->stats.device_bytes -= (int64_t)b.bytes;
+    cleanup_tmp();
+    if (e != hipSuccess)
+      return fail(LIBERI_EDEVICE, std::string("segment generation: ") + hipGetErrorString((hipError_t)e));
+    std::vector<int64_t> sstart;
+    std::vector<int32_t> sbra, sket0;
+    int64_t total = 0, primq = 0;
+    for (size_t i = 0; i < kb.size(); ++i)
+      for (int g = 0; g < ng; ++g) {
+        const int64_t L = len[i * ng + g];
+        if (L <= 0) continue;
+        sstart.push_back(total);
+        sbra.push_back(kb[i]);
+        sket0.push_back(gbeg[g]);
+        total += L;
+        primq += L * (int64_t)pairs[kb[i]].nprim * pairs[kk[gbeg[g]]].nprim;
+      }
+    sstart.push_back(total);
+    cs_.nquart = total;
+    cs_.nprimq = primq;
+    if (total == 0) continue;
+    cs_.nseg = (int32_t)sbra.size();
+    if ((rc = dev_upload(h, cs_.ket_list, kk)) || (rc = dev_upload(h, cs_.seg_start, sstart)) ||
+        (rc = dev_upload(h, cs_.seg_bra, sbra)) || (rc = dev_upload(h, cs_.seg_ket0, sket0)))
+      return rc;
+    cs_.active = true;
+    ClassDev &d = cs_.dev;
+    d.bra_list = nullptr;
+    d.ket_list = (const int32_t *)cs_.ket_list.p;
+    d.seg_start = (const int64_t *)cs_.seg_start.p;
+    d.seg_bra = (const int32_t *)cs_.seg_bra.p;
+    d.seg_ket0 = (const int32_t *)cs_.seg_ket0.p;
+    d.nseg = cs_.nseg;
+    d.diag = diag;
+    // this rank's contiguous share of the class (the reference splits each
+    // class's index range the same way, e.g. src/rhf/rot_axis/int0001.F90:100-104)
+    d.q_begin = total * my_rank / num_procs;
+    d.q_end = total * (my_rank + 1) / num_procs;
+    d.xswitch = h->exact_large_t ? class_rys_xmax(c) : reference_switch(c);
+    h->stats.quartets[c] = d.q_end - d.q_begin;
+    h->stats.prim_quartets[c] = primq;
+    h->stats.total_quartets += d.q_end - d.q_begin;
+    max_scratch = std::max(max_scratch, class_scratch_doubles(c));
+  }
+  h->scratch_per_wg = max_scratch;
+  if (max_scratch > 0)
+    if ((rc = dev_alloc(h, h->scratch, (size_t)max_scratch * kCoopWorkgroups * sizeof(double)))) return rc;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->initialized = true;
+  return LIBERI_OK;
+}
+
+// ------------------------------------------------------------------ build
+static int build_on_device(liberi_ctx *h, const double *dD, double *dF, hipStream_t st) {
+  const int64_t ntri = (int64_t)h->num_bas * (h->num_bas + 1) / 2;
+  HIP_TRY(hipMemsetAsync(dF, 0, ntri * sizeof(double), st));  // fock = 0 (src/liberi_driver.F90:64)
+  BuildArgs ba;
+  ba.pairs = (const PairRec *)h->pairs.p;
+  ba.prims = (const PrimRec *)h->prims.p;
+  ba.D = dD;
+  ba.F = dF;
+  ba.value_cut = h->value_cut;
+  ba.scratch = (double *)h->scratch.p;
+  ba.scratch_per_wg = h->scratch_per_wg;
+  HIP_TRY(hipEventRecord(h->ev0, st));
+  for (int c = 0; c < kNumClasses; ++c) {
+    if (!h->cls[c].active) continue;
+    int e = launch_class(c, h->cls[c].dev, ba, st, kCoopWorkgroups);
+    if (e != hipSuccess)
+      return fail(LIBERI_EDEVICE, std::string("launch ") + kClasses[c].name + ": " + hipGetErrorString((hipError_t)e));
+  }
+  HIP_TRY(hipEventRecord(h->ev1, st));
+  return LIBERI_OK;
+}
+
+int liberi_c_fock_build(liberi_ctx *h, const double *density, double *fock, int64_t n) {
+  if (!h) return fail(LIBERI_EARG, "liberi_c_fock_build: NULL handle");
+  if (!h->initialized)
+    return fail(LIBERI_ESTATE, "liberi_fock_build called before liberi_setup");  // liberi_interface.F90:219-222
+  const int64_t ntri = (int64_t)h->num_bas * (h->num_bas + 1) / 2;
+  if (n != ntri)
+    return fail(LIBERI_ESIZE, "liberi_c_fock_build: n=" + std::to_string(n) + ", expected " + std::to_string(ntri));
+  if (!density || !fock) return fail(LIBERI_EARG, "liberi_c_fock_build: NULL array");
+  HIP_TRY(hipMemcpyAsync(h->dens.p, density, ntri * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  int rc = build_on_device(h, (const double *)h->dens.p, (double *)h->fock.p, h->stream);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(fock, h->fock.p, ntri * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, h->ev0, h->ev1) == hipSuccess) h->stats.last_build_ms = ms;
+  return LIBERI_OK;
+}
+
+int liberi_c_fock_build_device(liberi_ctx *h, const double *dD, double *dF, int64_t n, void *stream) {
+  if (!h) return fail(LIBERI_EARG, "liberi_c_fock_build_device: NULL handle");
+  if (!h->initialized) return fail(LIBERI_ESTATE, "liberi_fock_build called before liberi_setup");
+  const int64_t ntri = (int64_t)h->num_bas * (h->num_bas + 1) / 2;
+  if (n != ntri)
+    return fail(LIBERI_ESIZE, "liberi_c_fock_build_device: n=" + std::to_string(n) + ", expected " + std::to_string(ntri));
+  if (!dD || !dF) return fail(LIBERI_EARG, "liberi_c_fock_build_device: NULL array");
+  return build_on_device(h, dD, dF, stream ? (hipStream_t)stream : h->stream);
+}
+
+// ------------------------------------------- GAMESS direct-link entry points
+static liberi_ctx *g_handle = nullptr;  // gms_liberi_wrapper.F90:23-25 (module `save` handle)
+static int64_t g_matrix_size = 0;
+
+int liberi_c_init(int32_t natoms, int32_t num_bas, int32_t nsh, int32_t mxgtot, const int32_t *ang_mom,
+                  const int32_t *contr_num, const int32_t *sh_loc, const int32_t *atom_num, const int32_t *atom_loc,
+                  const int32_t *start_bas, const int32_t *end_bas, const double *exponents, const double *cs,
+                  const double *cp, const double *cd, const double *cf, const double *coords, const double *schwrz,
+                  int32_t my_rank, int32_t num_procs) {
+  if (g_handle) liberi_c_destroy(&g_handle);  // gms_liberi_wrapper.F90:80-83
+  int rc = liberi_c_create(&g_handle);
+  if (rc) return rc;
+  rc = liberi_c_setup(g_handle, nsh, natoms, num_bas, mxgtot, ang_mom, contr_num, sh_loc, atom_num, atom_loc,
+                      start_bas, end_bas, exponents, cs, cp, cd, cf, coords, schwrz, my_rank, num_procs);
+  if (rc) {
+    std::string msg = g_last_error;
+    liberi_c_destroy(&g_handle);
+    return fail(rc, msg);
+  }
+  g_matrix_size = (int64_t)num_bas * (num_bas + 1) / 2;
+  return LIBERI_OK;
+}
+
+int liberi_c_fock(const double *density, double *fock, int64_t n) {
+  if (!g_handle) return fail(LIBERI_ESTATE, "liberi_fock called before liberi_init");  // :117-120
+  if (n != g_matrix_size)
+    return fail(LIBERI_ESIZE, "matrix size mismatch in liberi_fock: expected " + std::to_string(g_matrix_size) +
+                                  ", got " + std::to_string(n));  // :122-127
+  return liberi_c_fock_build(g_handle, density, fock, n);
+}
+
+int liberi_c_finalize(void) {
+  if (g_handle) liberi_c_destroy(&g_handle);
+  g_matrix_size = 0;
+  return LIBERI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,991 @@
+// Per-class ERI + Fock digestion kernels for gfx950 (CDNA4), FP64.
+//
+// Method: Rys quadrature (Rys, Dupuis & King, J. Comput. Chem. 4, 154 (1983))
+// on the primitive "(e0|f0)" integrals, e spanning angular momenta
+// la..la+lb and f lc..lc+ld, contracted over primitives, then the horizontal
+// recurrence (Head-Gordon & Pople, J. Chem. Phys. 89, 5777 (1988)) once per
+// contracted quartet.  Roots/weights come from piecewise polynomials in the
+// Rys argument X generated offline (gen/rys_tables.py).  This replaces the
+// reference's three kernel families (rotated-axis, ERIC, Rys; src/rhf/) with
+// one engine specialised per class at compile time.
+//
+// Two execution shapes:
+//  * lane-per-quartet (small classes): one lane owns one contracted quartet,
+//    all intermediates in registers, loops fully unrolled per class;
+//  * workgroup-per-quartet (large classes): the (e0|f0) accumulators are
+//    spread over the workgroup's lanes, the 2-D Rys tables of each primitive
+//    quartet are staged in LDS, the HRR runs on a per-workgroup scratch area.
+//
+// Digestion (the reference's DIRFCK_RHF, e.g. src/rhf/rot_axis/int0000.F90:241-311)
+// contracts each shell block with the six density blocks and adds the result
+// to the packed Fock matrix with one hardware FP64 atomic per Fock element of
+// the block (not one per integral).
+#include <hip/hip_runtime.h>
+
+#include "liberi_internal.hpp"
+
+#define RYS_CONST static __device__ const
+#include "rys_table.inc"
+// host copy of the end of the tabulated range per root count
+static const double rys_xmax_host[RYS_NMAX + 1] = {0.0, 39.0, 46.5, 53.5, 60.0, 66.0, 72.0, 77.5};
+
+namespace liberi {
+
+// ------------------------------------------------------------ cartesian maps
+__host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+__host__ __device__ constexpr int nrange(int l0, int l1) {
+  int n = 0;
+  for (int l = l0; l <= l1; ++l) n += ncart(l);
+  return n;
+}
+// internal order inside a shell of angular momentum L: i = L - lx, index i(i+1)/2 + lz
+__host__ __device__ constexpr int cidx(int lx, int ly, int lz) {
+  return ((ly + lz) * (ly + lz + 1)) / 2 + lz;
+}
+
+// component tables of the angular-momentum range [L0, L1] (internal order)
+template <int L0, int L1>
+struct Range {
+  static constexpr int n = nrange(L0, L1);
+  int lx[n], ly[n], lz[n], l[n];
+  constexpr Range() : lx(), ly(), lz(), l() {
+    int k = 0;
+    for (int L = L0; L <= L1; ++L)
+      for (int i = 0; i <= L; ++i)
+        for (int z = 0; z <= i; ++z) {
+          lx[k] = L - i;
+          ly[k] = i - z;
+          lz[k] = z;
+          l[k] = L;
+          ++k;
+        }
+  }
+};
+
+// GAMESS component order (p x,y,z; d xx,yy,zz,xy,xz,yz; f xxx,yyy,zzz,xxy,xxz,
+// xyy,yyz,xzz,yzz,xyz -- src/rhf/rys/int3300_rysgen.F90 ix/iy/iz tables) and the
+// kernels' scale factors (sqrt3 mixed d, sqrt5 / sqrt15 mixed f,
+// src/rhf/eric/int0030_ericgen.F90:49-57).
+__host__ __device__ constexpr int gamess_pos(int lx, int ly, int lz) {
+  const int L = lx + ly + lz;
+  if (L <= 1) return lx ? 0 : (ly ? 1 : (lz ? 2 : 0));
+  if (L == 2) {
+    if (lx == 2) return 0;
+    if (ly == 2) return 1;
+    if (lz == 2) return 2;
+    if (lx && ly) return 3;
+    if (lx && lz) return 4;
+    return 5;
+  }
+  if (lx == 3) return 0;
+  if (ly == 3) return 1;
+  if (lz == 3) return 2;
+  if (lx == 2) return ly ? 3 : 4;
+  if (ly == 2) return lx ? 5 : 6;
+  if (lz == 2) return lx ? 7 : 8;
+  return 9;
+}
+__host__ __device__ constexpr double cart_scale(int lx, int ly, int lz) {
+  const int L = lx + ly + lz;
+  const int mx = lx > ly ? (lx > lz ? lx : lz) : (ly > lz ? ly : lz);
+  if (L == 2) return mx == 2 ? 1.0 : 1.7320508075688772935;
+  if (L == 3) return mx == 3 ? 1.0 : (mx == 2 ? 2.2360679774997896964 : 3.8729833462074168852);
+  return 1.0;
+}
+
+__device__ __forceinline__ int64_t tri(int64_t i, int64_t j) {
+  return i >= j ? ((i * (i + 1)) >> 1) + j : ((j * (j + 1)) >> 1) + i;
+}
+
+// --------------------------------------------------------------- Rys rule
+// Tabulated rule for X < xswitch, asymptotic (Hermite) rule above.  xswitch
+// never exceeds rys_xmax[N]; the asymptotic rule reproduces exactly the
+// large-argument Boys values F_m = (2m-1)!!/(2X)^m sqrt(pi/4X) that the
+// reference uses past its switch points.
+template <int N>
+__device__ __forceinline__ void rys_rule(double X, double xswitch, double (&u)[N], double (&w)[N]) {
+  if (X < xswitch) {
+    const int j = (int)(X * RYS_INV_H);
+    const double s = X * (2.0 * RYS_INV_H) - (double)(2 * j + 1);
+    const double *c = rys_table + rys_offset[N] + j * (2 * N * (RYS_DEG + 1));
+#pragma unroll
+    for (int r = 0; r < 2 * N; ++r) {
+      const double *cr = c + r * (RYS_DEG + 1);
+      double acc = cr[RYS_DEG];
+#pragma unroll
+      for (int k = RYS_DEG - 1; k >= 0; --k) acc = fma(acc, s, cr[k]);
+      if (r < N)
+        u[r] = acc;
+      else
+        w[r - N] = acc;
+    }
+  } else {
+    const double ix = 1.0 / X, isx = sqrt(ix);
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+      u[r] = rys_herm_u[N][r] * ix;
+      w[r] = rys_herm_w[N][r] * isx;
+    }
+  }
+}
+
+// one root of the rule (used by the cooperative kernels: one lane per root)
+template <int N>
+__device__ __forceinline__ void rys_root(double X, double xswitch, int r, double &u, double &w) {
+  if (X < xswitch) {
+    const int j = (int)(X * RYS_INV_H);
+    const double s = X * (2.0 * RYS_INV_H) - (double)(2 * j + 1);
+    const double *c = rys_table + rys_offset[N] + j * (2 * N * (RYS_DEG + 1));
+    const double *cu = c + r * (RYS_DEG + 1), *cw = c + (N + r) * (RYS_DEG + 1);
+    double au = cu[RYS_DEG], aw = cw[RYS_DEG];
+#pragma unroll
+    for (int k = RYS_DEG - 1; k >= 0; --k) {
+      au = fma(au, s, cu[k]);
+      aw = fma(aw, s, cw[k]);
+    }
+    u = au;
+    w = aw;
+  } else {
+    const double ix = 1.0 / X;
+    u = rys_herm_u[N][r] * ix;
+    w = rys_herm_w[N][r] * sqrt(ix);
+  }
+}
+
+// 2-D Rys integrals G(i,k), i <= NI-1 (bra), k <= NK-1 (ket), for one root and
+// one cartesian direction:  G(i+1,0) = C00 G(i,0) + i B10 G(i-1,0),
+// G(i,k+1) = C00' G(i,k) + k B01 G(i,k-1) + i B00 G(i-1,k).
+template <int NI, int NK>
+__device__ __forceinline__ void rys_2d(double (&g)[NI][NK], double c00, double c00p, double b10, double b01,
+                                       double b00, double g00) {
+  g[0][0] = g00;
+  if (NI > 1) g[1][0] = c00 * g00;
+#pragma unroll
+  for (int i = 1; i + 1 < NI; ++i) g[i + 1][0] = c00 * g[i][0] + (double)i * b10 * g[i - 1][0];
+  if (NK > 1) {
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      double v = c00p * g[i][0];
+      if (i > 0) v += (double)i * b00 * g[i - 1][0];
+      g[i][1] = v;
+    }
+#pragma unroll
+    for (int k = 1; k + 1 < NK; ++k)
+#pragma unroll
+      for (int i = 0; i < NI; ++i) {
+        double v = c00p * g[i][k] + (double)k * b01 * g[i][k - 1];
+        if (i > 0) v += (double)i * b00 * g[i - 1][k];
+        g[i][k + 1] = v;
+      }
+  }
+}
+
+// quartet index -> (bra pair, ket pair) through the class's segment table
+__device__ __forceinline__ void decode_quartet(const ClassDev &cd, int64_t q, int32_t &bra, int32_t &ket) {
+  int lo = 0, hi = cd.nseg;  // seg_start[lo] <= q < seg_start[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (cd.seg_start[mid] <= q)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  bra = cd.seg_bra[lo];
+  ket = cd.ket_list[cd.seg_ket0[lo] + (int32_t)(q - cd.seg_start[lo])];
+}
+
+__device__ __forceinline__ void fock_add(double *F, int64_t i, int64_t j, double v) {
+  unsafeAtomicAdd(F + tri(i, j), v);
+}
+
+// ===================================================== lane-per-quartet path
+// size of the largest intermediate of the contracted HRR (both directions)
+__host__ __device__ constexpr int hrr_max_buffer(int LA, int LB, int LC, int LD) {
+  const int NF = nrange(LC, LC + LD);
+  int m = nrange(LA, LA + LB) * NF;
+  for (int j = 1; j <= LB; ++j) {
+    const int s = nrange(LA, LA + LB - j) * ncart(j) * NF;
+    m = s > m ? s : m;
+  }
+  for (int j = 1; j <= LD; ++j) {
+    const int s = ncart(LA) * ncart(LB) * nrange(LC, LC + LD - j) * ncart(j);
+    m = s > m ? s : m;
+  }
+  return m;
+}
+
+template <int LA, int LB, int LC, int LD>
+struct Small {
+  static constexpr int LAB = LA + LB, LCD = LC + LD;
+  static constexpr int NR = (LAB + LCD) / 2 + 1;
+  static constexpr int NE = nrange(LA, LAB), NF = nrange(LC, LCD);
+  static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  static constexpr int NBUF = hrr_max_buffer(LA, LB, LC, LD);
+};
+
+// bra HRR level J: out[a in range(LA, LAB-J)][b in cart(J)][f] from level J-1
+template <int LA, int LAB, int J, int NF>
+__device__ __forceinline__ void hrr_bra_level(const double *in, double *out, double abx, double aby, double abz) {
+  constexpr Range<LA, LAB - J> ra{};
+  constexpr Range<J, J> rb{};
+  constexpr Range<J - 1, J - 1> rbm{};
+  constexpr int nbm = ncart(J - 1), nb = ncart(J);
+  (void)rbm;
+#pragma unroll
+  for (int a = 0; a < ra.n; ++a)
+#pragma unroll
+    for (int b = 0; b < nb; ++b) {
+      const int bx = rb.lx[b], by = rb.ly[b], bz = rb.lz[b];
+      const int dir = bx ? 0 : (by ? 1 : 2);
+      const int bmi = cidx(bx - (dir == 0), by - (dir == 1), bz - (dir == 2));
+      const int ax = ra.lx[a] + (dir == 0), ay = ra.ly[a] + (dir == 1), az = ra.lz[a] + (dir == 2);
+      const int api = nrange(LA, ra.l[a]) + cidx(ax, ay, az);  // index in range(LA, LAB-J+1)
+      const double ab = dir == 0 ? abx : (dir == 1 ? aby : abz);
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+        out[(a * nb + b) * NF + f] = in[(api * nbm + bmi) * NF + f] + ab * in[(a * nbm + bmi) * NF + f];
+    }
+}
+
+template <int LA, int LB, int NF>
+__device__ __forceinline__ void hrr_bra(double *buf0, double *buf1, double abx, double aby, double abz) {
+  // level 0 lives in buf0 as [e in range(LA, LA+LB)][1][f]; result in buf0 when LB is even, buf1 when odd
+  if constexpr (LB >= 1) hrr_bra_level<LA, LA + LB, 1, NF>(buf0, buf1, abx, aby, abz);
+  if constexpr (LB >= 2) hrr_bra_level<LA, LA + LB, 2, NF>(buf1, buf0, abx, aby, abz);
+  if constexpr (LB >= 3) hrr_bra_level<LA, LA + LB, 3, NF>(buf0, buf1, abx, aby, abz);
+}
+
+// ket HRR level J on the inner index: out[ab][c in range(LC, LCD-J)][d in cart(J)]
+template <int LC, int LCD, int J, int NAB>
+__device__ __forceinline__ void hrr_ket_level(const double *in, double *out, double cdx, double cdy, double cdz) {
+  constexpr Range<LC, LCD - J> rc{};
+  constexpr Range<J, J> rd{};
+  constexpr int nin = nrange(LC, LCD - J + 1) * ncart(J - 1);
+  constexpr int nout = nrange(LC, LCD - J) * ncart(J);
+  constexpr int ndm = ncart(J - 1), nd = ncart(J);
+#pragma unroll
+  for (int c = 0; c < rc.n; ++c)
+#pragma unroll
+    for (int d = 0; d < nd; ++d) {
+      const int dx = rd.lx[d], dy = rd.ly[d], dz = rd.lz[d];
+      const int dir = dx ? 0 : (dy ? 1 : 2);
+      const int dmi = cidx(dx - (dir == 0), dy - (dir == 1), dz - (dir == 2));
+      const int cx = rc.lx[c] + (dir == 0), cy = rc.ly[c] + (dir == 1), cz = rc.lz[c] + (dir == 2);
+      const int cpi = nrange(LC, rc.l[c]) + cidx(cx, cy, cz);
+      const double cdv = dir == 0 ? cdx : (dir == 1 ? cdy : cdz);
+#pragma unroll
+      for (int ab = 0; ab < NAB; ++ab)
+        out[ab * nout + c * nd + d] = in[ab * nin + cpi * ndm + dmi] + cdv * in[ab * nin + c * ndm + dmi];
+    }
+}
+
+template <int LC, int LD, int NAB>
+__device__ __forceinline__ void hrr_ket(double *buf0, double *buf1, double cdx, double cdy, double cdz) {
+  if constexpr (LD >= 1) hrr_ket_level<LC, LC + LD, 1, NAB>(buf0, buf1, cdx, cdy, cdz);
+  if constexpr (LD >= 2) hrr_ket_level<LC, LC + LD, 2, NAB>(buf1, buf0, cdx, cdy, cdz);
+  if constexpr (LD >= 3) hrr_ket_level<LC, LC + LD, 3, NAB>(buf0, buf1, cdx, cdy, cdz);
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void eri_small_kernel(ClassDev cd, BuildArgs ba) {
+  using S = Small<LA, LB, LC, LD>;
+  constexpr int NE = S::NE, NF = S::NF, NR = S::NR, NI = S::LAB + 1, NK = S::LCD + 1;
+  constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND;
+  constexpr int NBUF = S::NBUF;
+  constexpr Range<LA, LA + LB> re{};
+  constexpr Range<LC, LC + LD> rf{};
+
+  const int64_t q = cd.q_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= cd.q_end) return;
+  int32_t ib, ik;
+  decode_quartet(cd, q, ib, ik);
+  const PairRec bra = ba.pairs[ib];
+  const PairRec ket = ba.pairs[ik];
+
+  double acc[NBUF];
+#pragma unroll
+  for (int i = 0; i < NE * NF; ++i) acc[i] = 0.0;
+
+  for (int ip = 0; ip < bra.nprim; ++ip) {
+    const PrimRec P = ba.prims[bra.prim_off + ip];
+    for (int iq = 0; iq < ket.nprim; ++iq) {
+      const PrimRec Q = ba.prims[ket.prim_off + iq];
+      const double pq = P.p + Q.p, ipq = 1.0 / pq;
+      const double pqx = P.Px - Q.Px, pqy = P.Py - Q.Py, pqz = P.Pz - Q.Pz;
+      const double rho = P.p * Q.p * ipq;
+      const double X = rho * (pqx * pqx + pqy * pqy + pqz * pqz);
+      const double pref = kTwoPi52 * P.K * Q.K * sqrt(ipq);
+      double u[NR], w[NR];
+      rys_rule<NR>(X, cd.xswitch, u, w);
+      const double hp = 0.5 / P.p, hq = 0.5 / Q.p;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const double t2 = u[r];
+        const double b00 = 0.5 * t2 * ipq;
+        const double b10 = hp - hp * Q.p * t2 * ipq;
+        const double b01 = hq - hq * P.p * t2 * ipq;
+        const double fa = Q.p * ipq * t2, fc = P.p * ipq * t2;
+        double gx[NI][NK], gy[NI][NK], gz[NI][NK];
+        rys_2d<NI, NK>(gx, P.PAx - fa * pqx, Q.PAx + fc * pqx, b10, b01, b00, 1.0);
+        rys_2d<NI, NK>(gy, P.PAy - fa * pqy, Q.PAy + fc * pqy, b10, b01, b00, 1.0);
+        rys_2d<NI, NK>(gz, P.PAz - fa * pqz, Q.PAz + fc * pqz, b10, b01, b00, w[r] * pref);
+#pragma unroll
+        for (int e = 0; e < NE; ++e)
+#pragma unroll
+          for (int f = 0; f < NF; ++f)
+            acc[e * NF + f] += gx[re.lx[e]][rf.lx[f]] * gy[re.ly[e]][rf.ly[f]] * gz[re.lz[e]][rf.lz[f]];
+      }
+    }
+  }
+
+  // contracted HRR: bra then ket (in-place ping-pong through two register arrays)
+  double b1[NBUF];
+  double *res = acc;
+  if constexpr (LB > 0) {
+    hrr_bra<LA, LB, NF>(acc, b1, bra.ABx, bra.ABy, bra.ABz);
+    res = (LB % 2) ? b1 : acc;
+  }
+  if constexpr (LD > 0) {
+    double *other = (res == acc) ? b1 : acc;
+    hrr_ket<LC, LD, NA * NB>(res, other, ket.ABx, ket.ABy, ket.ABz);
+    if (LD % 2) res = other;
+  }
+
+  // normalised block v[a][b][c][d] (internal order), value cutoff
+  constexpr Range<LA, LA> ra{};
+  constexpr Range<LB, LB> rb{};
+  constexpr Range<LC, LC> rc{};
+  constexpr Range<LD, LD> rd{};
+  const double cut = ba.value_cut;
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+          const double sc = cart_scale(ra.lx[a], ra.ly[a], ra.lz[a]) * cart_scale(rb.lx[b], rb.ly[b], rb.lz[b]) *
+                            cart_scale(rc.lx[c], rc.ly[c], rc.lz[c]) * cart_scale(rd.lx[d], rd.ly[d], rd.lz[d]);
+          const int idx = ((a * NB + b) * NC + c) * ND + d;
+          const double v = res[idx] * sc;
+          res[idx] = fabs(v) < cut ? 0.0 : v;
+        }
+
+  // degeneracy weight of the shell quartet (A==B, C==D, AB==CD)
+  double s = 1.0;
+  if (bra.sameAB) s *= 0.5;
+  if (ket.sameAB) s *= 0.5;
+  if (cd.diag && ib == ik) s *= 0.5;
+
+  int ga[NA], gb[NB], gc[NC], gd[ND];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) ga[a] = bra.bfA + gamess_pos(ra.lx[a], ra.ly[a], ra.lz[a]);
+#pragma unroll
+  for (int b = 0; b < NB; ++b) gb[b] = bra.bfB + gamess_pos(rb.lx[b], rb.ly[b], rb.lz[b]);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) gc[c] = ket.bfA + gamess_pos(rc.lx[c], rc.ly[c], rc.lz[c]);
+#pragma unroll
+  for (int d = 0; d < ND; ++d) gd[d] = ket.bfB + gamess_pos(rd.lx[d], rd.ly[d], rd.lz[d]);
+
+  const double *Dm = ba.D;
+  double *F = ba.F;
+  // J: F[ab] += 4s sum_cd v D[cd], F[cd] += 4s sum_ab v D[ab]
+  {
+    double dcd[NC * ND];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int d = 0; d < ND; ++d) dcd[c * ND + d] = Dm[tri(gc[c], gd[d])];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < NC * ND; ++k) t += res[(a * NB + b) * NC * ND + k] * dcd[k];
+        if (t != 0.0) fock_add(F, ga[a], gb[b], 4.0 * s * t);
+      }
+  }
+  {
+    double dab[NA * NB];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) dab[a * NB + b] = Dm[tri(ga[a], gb[b])];
+#pragma unroll
+    for (int k = 0; k < NC * ND; ++k) {
+      double t = 0.0;
+#pragma unroll
+      for (int ab = 0; ab < NA * NB; ++ab) t += res[ab * NC * ND + k] * dab[ab];
+      if (t != 0.0) fock_add(F, gc[k / ND], gd[k % ND], 4.0 * s * t);
+    }
+  }
+  // K: F[ac] -= s sum_bd v D[bd];  F[bd] -= s sum_ac v D[ac]
+  {
+    double dbd[NB * ND], dac[NA * NC];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int d = 0; d < ND; ++d) dbd[b * ND + d] = Dm[tri(gb[b], gd[d])];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) dac[a * NC + c] = Dm[tri(ga[a], gc[c])];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        double t = 0.0;
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+          for (int d = 0; d < ND; ++d) t += res[((a * NB + b) * NC + c) * ND + d] * dbd[b * ND + d];
+        if (t != 0.0) fock_add(F, ga[a], gc[c], -s * t);
+      }
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        double t = 0.0;
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+          for (int c = 0; c < NC; ++c) t += res[((a * NB + b) * NC + c) * ND + d] * dac[a * NC + c];
+        if (t != 0.0) fock_add(F, gb[b], gd[d], -s * t);
+      }
+  }
+  // K: F[ad] -= s sum_bc v D[bc];  F[bc] -= s sum_ad v D[ad]
+  {
+    double dbc[NB * NC], dad[NA * ND];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) dbc[b * NC + c] = Dm[tri(gb[b], gc[c])];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int d = 0; d < ND; ++d) dad[a * ND + d] = Dm[tri(ga[a], gd[d])];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        double t = 0.0;
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+          for (int c = 0; c < NC; ++c) t += res[((a * NB + b) * NC + c) * ND + d] * dbc[b * NC + c];
+        if (t != 0.0) fock_add(F, ga[a], gd[d], -s * t);
+      }
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        double t = 0.0;
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+          for (int d = 0; d < ND; ++d) t += res[((a * NB + b) * NC + c) * ND + d] * dad[a * ND + d];
+        if (t != 0.0) fock_add(F, gb[b], gc[c], -s * t);
+      }
+  }
+}
+
+// ================================================ workgroup-per-quartet path
+__device__ __forceinline__ void cart_decode(int L, int m, int &lx, int &ly, int &lz) {
+  int i = 0;
+  while ((i + 1) * (i + 2) / 2 <= m) ++i;
+  lz = m - i * (i + 1) / 2;
+  lx = L - i;
+  ly = i - lz;
+}
+// component k of range [L0, ...] -> its angular momentum and (lx,ly,lz)
+__device__ __forceinline__ void range_decode(int L0, int k, int &L, int &lx, int &ly, int &lz) {
+  L = L0;
+  while (k >= ncart(L)) {
+    k -= ncart(L);
+    ++L;
+  }
+  cart_decode(L, k, lx, ly, lz);
+}
+__device__ __forceinline__ int range_index(int L0, int lx, int ly, int lz) {
+  return nrange(L0, lx + ly + lz - 1) + cidx(lx, ly, lz);
+}
+
+template <int LA, int LB, int LC, int LD>
+struct Coop {
+  static constexpr int LAB = LA + LB, LCD = LC + LD;
+  static constexpr int NR = (LAB + LCD) / 2 + 1;
+  static constexpr int NI = LAB + 1, NK = LCD + 1;
+  static constexpr int NE = nrange(LA, LAB), NF = nrange(LC, LCD);
+  static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  static constexpr int NABCD = NA * NB * NC * ND;
+  static constexpr int NT = NE * NF > 2048 ? 256 : (NE * NF > 512 ? 128 : 64);
+  static constexpr int NKC = (NE * NF + NT - 1) / NT;  // accumulators per lane
+  static constexpr int64_t buf_doubles() {
+    int64_t m = (int64_t)NE * NF;
+    for (int j = 1; j <= LB; ++j) {
+      int64_t s = (int64_t)nrange(LA, LAB - j) * ncart(j) * NF;
+      m = s > m ? s : m;
+    }
+    for (int j = 1; j <= LD; ++j) {
+      int64_t s = (int64_t)NA * NB * nrange(LC, LCD - j) * ncart(j);
+      m = s > m ? s : m;
+    }
+    return m;
+  }
+  static constexpr int64_t scratch() { return 2 * buf_doubles(); }
+};
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__((Coop<LA, LB, LC, LD>::NT)) void eri_coop_kernel(ClassDev cd, BuildArgs ba) {
+  using S = Coop<LA, LB, LC, LD>;
+  constexpr int NT = S::NT, NR = S::NR, NI = S::NI, NK = S::NK, NE = S::NE, NF = S::NF, NKC = S::NKC;
+  constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND, NABCD = S::NABCD;
+  constexpr int GSZ = NI * NK;
+  __shared__ double G[3][NR][GSZ];
+
+  const int t = threadIdx.x;
+  double *buf0 = ba.scratch + (int64_t)blockIdx.x * ba.scratch_per_wg;
+  double *buf1 = buf0 + S::buf_doubles();
+
+  // per-lane component offsets into the G tables (fixed for the kernel)
+  int offx[NKC], offy[NKC], offz[NKC];
+#pragma unroll
+  for (int k = 0; k < NKC; ++k) {
+    const int c = t + NT * k;
+    if (c < NE * NF) {
+      int Le, ex, ey, ez, Lf, fx, fy, fz;
+      range_decode(LA, c / NF, Le, ex, ey, ez);
+      range_decode(LC, c % NF, Lf, fx, fy, fz);
+      offx[k] = ex * NK + fx;
+      offy[k] = ey * NK + fy;
+      offz[k] = ez * NK + fz;
+    } else {
+      offx[k] = offy[k] = offz[k] = 0;
+    }
+  }
+
+  for (int64_t q = cd.q_begin + blockIdx.x; q < cd.q_end; q += gridDim.x) {
+    int32_t ib, ik;
+    decode_quartet(cd, q, ib, ik);
+    const PairRec bra = ba.pairs[ib];
+    const PairRec ket = ba.pairs[ik];
+    double acc[NKC];
+#pragma unroll
+    for (int k = 0; k < NKC; ++k) acc[k] = 0.0;
+
+    for (int ip = 0; ip < bra.nprim; ++ip) {
+      for (int iq = 0; iq < ket.nprim; ++iq) {
+        if (t < 3 * NR) {
+          const PrimRec P = ba.prims[bra.prim_off + ip];
+          const PrimRec Q = ba.prims[ket.prim_off + iq];
+          const int r = t / 3, dim = t - 3 * (t / 3);
+          const double pq = P.p + Q.p, ipq = 1.0 / pq;
+          const double pqx = P.Px - Q.Px, pqy = P.Py - Q.Py, pqz = P.Pz - Q.Pz;
+          const double rho = P.p * Q.p * ipq;
+          const double X = rho * (pqx * pqx + pqy * pqy + pqz * pqz);
+          double t2, wr;
+          rys_root<NR>(X, cd.xswitch, r, t2, wr);
+          const double hp = 0.5 / P.p, hq = 0.5 / Q.p;
+          const double b00 = 0.5 * t2 * ipq;
+          const double b10 = hp - hp * Q.p * t2 * ipq;
+          const double b01 = hq - hq * P.p * t2 * ipq;
+          const double fa = Q.p * ipq * t2, fc = P.p * ipq * t2;
+          const double pqd = dim == 0 ? pqx : (dim == 1 ? pqy : pqz);
+          const double pa = dim == 0 ? P.PAx : (dim == 1 ? P.PAy : P.PAz);
+          const double qc = dim == 0 ? Q.PAx : (dim == 1 ? Q.PAy : Q.PAz);
+          const double g00 = dim == 2 ? wr * kTwoPi52 * P.K * Q.K * sqrt(ipq) : 1.0;
+          double g[NI][NK];
+          rys_2d<NI, NK>(g, pa - fa * pqd, qc + fc * pqd, b10, b01, b00, g00);
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+#pragma unroll
+            for (int k = 0; k < NK; ++k) G[dim][r][i * NK + k] = g[i][k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NKC; ++k) {
+          double v = 0.0;
+#pragma unroll
+          for (int r = 0; r < NR; ++r) v += G[0][r][offx[k]] * G[1][r][offy[k]] * G[2][r][offz[k]];
+          acc[k] += v;
+        }
+        __syncthreads();
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NKC; ++k) {
+      const int c = t + NT * k;
+      if (c < NE * NF) buf0[c] = acc[k];
+    }
+    __syncthreads();
+
+    // bra HRR: level j out[a in range(LA, LAB-j)][b in cart(j)][f]
+    double *in = buf0, *out = buf1;
+    for (int j = 1; j <= LB; ++j) {
+      const int na = nrange(LA, S::LAB - j), nb = ncart(j), nbm = ncart(j - 1);
+      const int items = na * nb * NF;
+      for (int it = t; it < items; it += NT) {
+        const int f = it % NF, rest = it / NF, b = rest % nb, a = rest / nb;
+        int bx, by, bz, La, ax, ay, az;
+        cart_decode(j, b, bx, by, bz);
+        range_decode(LA, a, La, ax, ay, az);
+        const int dir = bx ? 0 : (by ? 1 : 2);
+        const int bmi = cidx(bx - (dir == 0), by - (dir == 1), bz - (dir == 2));
+        const int api = range_index(LA, ax + (dir == 0), ay + (dir == 1), az + (dir == 2));
+        const double abv = dir == 0 ? bra.ABx : (dir == 1 ? bra.ABy : bra.ABz);
+        out[(a * nb + b) * NF + f] = in[(api * nbm + bmi) * NF + f] + abv * in[(a * nbm + bmi) * NF + f];
+      }
+      __syncthreads();
+      double *tmp = in;
+      in = out;
+      out = tmp;
+    }
+    // ket HRR on the inner index
+    for (int j = 1; j <= LD; ++j) {
+      const int nc = nrange(LC, S::LCD - j), nd = ncart(j), ndm = ncart(j - 1);
+      const int nin = nrange(LC, S::LCD - j + 1) * ndm, nout = nc * nd;
+      const int items = NA * NB * nout;
+      for (int it = t; it < items; it += NT) {
+        const int ab = it / nout, rest = it % nout, c = rest / nd, d = rest % nd;
+        int dx, dy, dz, Lc, cx, cy, cz;
+        cart_decode(j, d, dx, dy, dz);
+        range_decode(LC, c, Lc, cx, cy, cz);
+        const int dir = dx ? 0 : (dy ? 1 : 2);
+        const int dmi = cidx(dx - (dir == 0), dy - (dir == 1), dz - (dir == 2));
+        const int cpi = range_index(LC, cx + (dir == 0), cy + (dir == 1), cz + (dir == 2));
+        const double cdv = dir == 0 ? ket.ABx : (dir == 1 ? ket.ABy : ket.ABz);
+        out[ab * nout + c * nd + d] = in[ab * nin + cpi * ndm + dmi] + cdv * in[ab * nin + c * ndm + dmi];
+      }
+      __syncthreads();
+      double *tmp = in;
+      in = out;
+      out = tmp;
+    }
+    // scale + value cutoff, in place: v[a][b][c][d]
+    for (int it = t; it < NABCD; it += NT) {
+      const int d = it % ND, c = (it / ND) % NC, b = (it / (ND * NC)) % NB, a = it / (ND * NC * NB);
+      int x0, y0, z0, x1, y1, z1, x2, y2, z2, x3, y3, z3;
+      cart_decode(LA, a, x0, y0, z0);
+      cart_decode(LB, b, x1, y1, z1);
+      cart_decode(LC, c, x2, y2, z2);
+      cart_decode(LD, d, x3, y3, z3);
+      const double v = in[it] * cart_scale(x0, y0, z0) * cart_scale(x1, y1, z1) * cart_scale(x2, y2, z2) *
+                       cart_scale(x3, y3, z3);
+      in[it] = fabs(v) < ba.value_cut ? 0.0 : v;
+    }
+    __syncthreads();
+
+    double s = 1.0;
+    if (bra.sameAB) s *= 0.5;
+    if (ket.sameAB) s *= 0.5;
+    if (cd.diag && ib == ik) s *= 0.5;
+    const double *v = in;
+    const double *Dm = ba.D;
+    double *F = ba.F;
+    // six contracted blocks; one output element per lane iteration
+    constexpr int n1 = NA * NB, n2 = NC * ND, n3 = NA * NC, n4 = NB * ND, n5 = NA * ND, n6 = NB * NC;
+    for (int it = t; it < n1 + n2 + n3 + n4 + n5 + n6; it += NT) {
+      int x, y, z;
+      double acc2 = 0.0;
+      int64_t fi, fj;
+      double fac;
+      if (it < n1) {  // F[ab] += 4s sum_cd v D[cd]
+        const int a = it / NB, b = it % NB;
+        cart_decode(LA, a, x, y, z);
+        fi = bra.bfA + gamess_pos(x, y, z);
+        cart_decode(LB, b, x, y, z);
+        fj = bra.bfB + gamess_pos(x, y, z);
+        for (int c = 0; c < NC; ++c) {
+          cart_decode(LC, c, x, y, z);
+          const int64_t gc = ket.bfA + gamess_pos(x, y, z);
+          for (int d = 0; d < ND; ++d) {
+            cart_decode(LD, d, x, y, z);
+            acc2 += v[((a * NB + b) * NC + c) * ND + d] * Dm[tri(gc, ket.bfB + gamess_pos(x, y, z))];
+          }
+        }
+        fac = 4.0 * s;
+      } else if (it < n1 + n2) {  // F[cd] += 4s sum_ab v D[ab]
+        const int k = it - n1, c = k / ND, d = k % ND;
+        cart_decode(LC, c, x, y, z);
+        fi = ket.bfA + gamess_pos(x, y, z);
+        cart_decode(LD, d, x, y, z);
+        fj = ket.bfB + gamess_pos(x, y, z);
+        for (int a = 0; a < NA; ++a) {
+          cart_decode(LA, a, x, y, z);
+          const int64_t ga = bra.bfA + gamess_pos(x, y, z);
+          for (int b = 0; b < NB; ++b) {
+            cart_decode(LB, b, x, y, z);
+            acc2 += v[((a * NB + b) * NC + c) * ND + d] * Dm[tri(ga, bra.bfB + gamess_pos(x, y, z))];
+          }
+        }
+        fac = 4.0 * s;
+      } else if (it < n1 + n2 + n3) {  // F[ac] -= s sum_bd v D[bd]
+        const int k = it - n1 - n2, a = k / NC, c = k % NC;
+        cart_decode(LA, a, x, y, z);
+        fi = bra.bfA + gamess_pos(x, y, z);
+        cart_decode(LC, c, x, y, z);
+        fj = ket.bfA + gamess_pos(x, y, z);
+        for (int b = 0; b < NB; ++b) {
+          cart_decode(LB, b, x, y, z);
+          const int64_t gb = bra.bfB + gamess_pos(x, y, z);
+          for (int d = 0; d < ND; ++d) {
+            cart_decode(LD, d, x, y, z);
+            acc2 += v[((a * NB + b) * NC + c) * ND + d] * Dm[tri(gb, ket.bfB + gamess_pos(x, y, z))];
+          }
+        }
+        fac = -s;
+      } else if (it < n1 + n2 + n3 + n4) {  // F[bd] -= s sum_ac v D[ac]
+        const int k = it - n1 - n2 - n3, b = k / ND, d = k % ND;
+        cart_decode(LB, b, x, y, z);
+        fi = bra.bfB + gamess_pos(x, y, z);
+        cart_decode(LD, d, x, y, z);
+        fj = ket.bfB + gamess_pos(x, y, z);
+        for (int a = 0; a < NA; ++a) {
+          cart_decode(LA, a, x, y, z);
+          const int64_t ga = bra.bfA + gamess_pos(x, y, z);
+          for (int c = 0; c < NC; ++c) {
+            cart_decode(LC, c, x, y, z);
+            acc2 += v[((a * NB + b) * NC + c) * ND + d] * Dm[tri(ga, ket.bfA + gamess_pos(x, y, z))];
+          }
+        }
+        fac = -s;
+      } else if (it < n1 + n2 + n3 + n4 + n5) {  // F[ad] -= s sum_bc v D[bc]
+        const int k = it - n1 - n2 - n3 - n4, a = k / ND, d = k % ND;
+        cart_decode(LA, a, x, y, z);
+        fi = bra.bfA + gamess_pos(x, y, z);
+        cart_decode(LD, d, x, y, z);
+        fj = ket.bfB + gamess_pos(x, y, z);
+        for (int b = 0; b < NB; ++b) {
+          cart_decode(LB, b, x, y, z);
+          const int64_t gb = bra.bfB + gamess_pos(x, y, z);
+          for (int c = 0; c < NC; ++c) {
+            cart_decode(LC, c, x, y, z);
+            acc2 += v[((a * NB + b) * NC + c) * ND + d] * Dm[tri(gb, ket.bfA + gamess_pos(x, y, z))];
+          }
+        }
+        fac = -s;
+      } else {  // F[bc] -= s sum_ad v D[ad]
+        const int k = it - n1 - n2 - n3 - n4 - n5, b = k / NC, c = k % NC;
+        cart_decode(LB, b, x, y, z);
+        fi = bra.bfB + gamess_pos(x, y, z);
+        cart_decode(LC, c, x, y, z);
+        fj = ket.bfA + gamess_pos(x, y, z);
+        for (int a = 0; a < NA; ++a) {
+          cart_decode(LA, a, x, y, z);
+          const int64_t ga = bra.bfA + gamess_pos(x, y, z);
+          for (int d = 0; d < ND; ++d) {
+            cart_decode(LD, d, x, y, z);
+            acc2 += v[((a * NB + b) * NC + c) * ND + d] * Dm[tri(ga, ket.bfB + gamess_pos(x, y, z))];
+          }
+        }
+        fac = -s;
+      }
+      if (acc2 != 0.0) fock_add(F, fi, fj, fac * acc2);
+    }
+    __syncthreads();  // scratch is reused by the next quartet
+  }
+}
+
+// ----------------------------------------------------- segment lengths
+#if !defined(LIBERI_CLASS_GROUP) || LIBERI_CLASS_GROUP < 0
+__global__ void segment_lengths_kernel(const double *bra_x, const int32_t *bra_group, const int32_t *bra_pos,
+                                       int32_t nbra, const double *ket_x, const int32_t *gbeg, const int32_t *gend,
+                                       int32_t ngroup, int32_t diag, int32_t cmp, double cut, int64_t *seg_len) {
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (int64_t)nbra * ngroup) return;
+  const int i = (int)(id / ngroup), g = (int)(id % ngroup);
+  const double xi = bra_x[i];
+  int lo = gbeg[g], hi = gend[g];
+  int limit = hi;
+  if (diag) {
+    const int gi = bra_group[i];
+    if (g > gi) {
+      seg_len[id] = 0;
+      return;
+    }
+    if (g == gi) limit = bra_pos[i] + 1;  // kets up to and including the bra itself
+  }
+  // first position in [lo, hi) whose product fails the test (x descending)
+  int a = lo, b = hi;
+  if (cmp != CMP_NONE) {
+    while (a < b) {
+      const int m = (a + b) >> 1;
+      const double tst = xi * ket_x[m];
+      const bool pass = cmp == CMP_GT ? (tst > cut) : !(tst < cut);
+      if (pass)
+        a = m + 1;
+      else
+        b = m;
+    }
+  } else {
+    a = hi;
+  }
+  const int end = a < limit ? a : limit;
+  seg_len[id] = end > lo ? end - lo : 0;
+}
+
+int launch_segment_lengths(const double *bra_x, const int32_t *bra_group, const int32_t *bra_pos, int32_t nbra,
+                           const double *ket_x, const int32_t *gbeg, const int32_t *gend, int32_t ngroup,
+                           int32_t diag, int32_t cmp, double cut, int64_t *seg_len, void *stream) {
+  const int64_t n = (int64_t)nbra * ngroup;
+  if (n == 0) return hipSuccess;
+  const int tpb = 256;
+  hipLaunchKernelGGL(segment_lengths_kernel, dim3((unsigned)((n + tpb - 1) / tpb)), dim3(tpb), 0,
+                     (hipStream_t)stream, bra_x, bra_group, bra_pos, nbra, ket_x, gbeg, gend, ngroup, diag, cmp,
+                     cut, seg_len);
+  return hipGetLastError();
+}
+#endif
+
+// ----------------------------------------------------------- dispatch
+template <int LA, int LB, int LC, int LD>
+constexpr bool use_small() {
+  return nrange(LA, LA + LB) * nrange(LC, LC + LD) <= 100 && hrr_max_buffer(LA, LB, LC, LD) <= 100;
+}
+
+template <int LA, int LB, int LC, int LD>
+static int launch_one(const ClassDev &cd, const BuildArgs &ba, hipStream_t st, int nwg) {
+  const int64_t n = cd.q_end - cd.q_begin;
+  if (n <= 0) return hipSuccess;
+  if constexpr (use_small<LA, LB, LC, LD>()) {
+    const int tpb = 256;
+    hipLaunchKernelGGL((eri_small_kernel<LA, LB, LC, LD>), dim3((unsigned)((n + tpb - 1) / tpb)), dim3(tpb), 0,
+                       st, cd, ba);
+  } else {
+    using S = Coop<LA, LB, LC, LD>;
+    const int64_t g = n < nwg ? n : nwg;
+    hipLaunchKernelGGL((eri_coop_kernel<LA, LB, LC, LD>), dim3((unsigned)g), dim3(S::NT), 0, st, cd, ba);
+  }
+  return hipGetLastError();
+}
+
+template <int LA, int LB, int LC, int LD>
+static int64_t scratch_one() {
+  if constexpr (use_small<LA, LB, LC, LD>())
+    return 0;
+  else
+    return Coop<LA, LB, LC, LD>::scratch();
+}
+
+// class c -> angular momenta (A >= B within a pair: the higher-L shell first)
+#define LIBERI_CLASS_LIST(X)                                                                              \
+  X(0, 0, 0, 0, 0) X(1, 0, 0, 1, 0) X(2, 0, 0, 1, 1) X(3, 1, 0, 1, 1) X(4, 1, 0, 1, 0) X(5, 1, 1, 1, 1)   \
+  X(6, 0, 0, 2, 0) X(7, 0, 0, 2, 1) X(8, 1, 0, 2, 0) X(9, 1, 0, 2, 1) X(10, 0, 0, 2, 2)                   \
+  X(11, 1, 0, 2, 2) X(12, 1, 1, 2, 0) X(13, 2, 0, 2, 0) X(14, 2, 2, 1, 1) X(15, 2, 1, 2, 1)               \
+  X(16, 2, 1, 2, 0) X(17, 2, 1, 1, 1) X(18, 2, 2, 2, 0) X(19, 2, 2, 2, 1) X(20, 2, 2, 2, 2)               \
+  X(21, 3, 3, 3, 3) X(22, 0, 0, 3, 0) X(23, 0, 0, 3, 1) X(24, 0, 0, 3, 2) X(25, 1, 0, 3, 0)               \
+  X(26, 3, 3, 3, 0) X(27, 3, 0, 1, 1) X(28, 3, 0, 2, 0) X(29, 2, 1, 3, 0) X(30, 2, 2, 3, 0)               \
+  X(31, 3, 0, 3, 0) X(32, 3, 1, 1, 0) X(33, 3, 1, 1, 1) X(34, 3, 1, 2, 0) X(35, 3, 1, 2, 1)               \
+  X(36, 2, 2, 3, 1) X(37, 3, 1, 3, 0) X(38, 3, 1, 3, 1) X(39, 3, 2, 1, 0) X(40, 3, 3, 3, 1)               \
+  X(41, 3, 2, 1, 1) X(42, 3, 2, 2, 0) X(43, 3, 2, 2, 1) X(44, 3, 2, 2, 2) X(45, 3, 2, 3, 0)               \
+  X(46, 3, 2, 3, 1) X(47, 3, 2, 3, 2) X(48, 3, 3, 0, 0) X(49, 3, 3, 1, 0) X(50, 3, 3, 1, 1)               \
+  X(51, 3, 3, 2, 0) X(52, 3, 3, 2, 1) X(53, 3, 3, 2, 2) X(54, 3, 3, 3, 2)
+
+// The 55 instantiations are spread over LIBERI_CLASS_GROUPS translation units
+// (class id modulo the group count) so they compile in parallel; the unit
+// built with LIBERI_CLASS_GROUP=-1 holds the dispatcher.
+#define LIBERI_CLASS_LIST_G0(X) X(0, 0, 0, 0, 0) X(8, 1, 0, 2, 0) X(16, 2, 1, 2, 0) X(24, 0, 0, 3, 2) X(32, 3, 1, 1, 0) X(40, 3, 3, 3, 1) X(48, 3, 3, 0, 0)
+#define LIBERI_CLASS_LIST_G1(X) X(1, 0, 0, 1, 0) X(9, 1, 0, 2, 1) X(17, 2, 1, 1, 1) X(25, 1, 0, 3, 0) X(33, 3, 1, 1, 1) X(41, 3, 2, 1, 1) X(49, 3, 3, 1, 0)
+#define LIBERI_CLASS_LIST_G2(X) X(2, 0, 0, 1, 1) X(10, 0, 0, 2, 2) X(18, 2, 2, 2, 0) X(26, 3, 3, 3, 0) X(34, 3, 1, 2, 0) X(42, 3, 2, 2, 0) X(50, 3, 3, 1, 1)
+#define LIBERI_CLASS_LIST_G3(X) X(3, 1, 0, 1, 1) X(11, 1, 0, 2, 2) X(19, 2, 2, 2, 1) X(27, 3, 0, 1, 1) X(35, 3, 1, 2, 1) X(43, 3, 2, 2, 1) X(51, 3, 3, 2, 0)
+#define LIBERI_CLASS_LIST_G4(X) X(4, 1, 0, 1, 0) X(12, 1, 1, 2, 0) X(20, 2, 2, 2, 2) X(28, 3, 0, 2, 0) X(36, 2, 2, 3, 1) X(44, 3, 2, 2, 2) X(52, 3, 3, 2, 1)
+#define LIBERI_CLASS_LIST_G5(X) X(5, 1, 1, 1, 1) X(13, 2, 0, 2, 0) X(21, 3, 3, 3, 3) X(29, 2, 1, 3, 0) X(37, 3, 1, 3, 0) X(45, 3, 2, 3, 0) X(53, 3, 3, 2, 2)
+#define LIBERI_CLASS_LIST_G6(X) X(6, 0, 0, 2, 0) X(14, 2, 2, 1, 1) X(22, 0, 0, 3, 0) X(30, 2, 2, 3, 0) X(38, 3, 1, 3, 1) X(46, 3, 2, 3, 1) X(54, 3, 3, 3, 2)
+#define LIBERI_CLASS_LIST_G7(X) X(7, 0, 0, 2, 1) X(15, 2, 1, 2, 1) X(23, 0, 0, 3, 1) X(31, 3, 0, 3, 0) X(39, 3, 2, 1, 0) X(47, 3, 2, 3, 2)
+
+#define DECL_CLASS(id, a, b, cc, d)                                                       \
+  int launch_class_##id(const ClassDev &cd, const BuildArgs &ba, hipStream_t st, int nwg); \
+  int64_t scratch_class_##id();
+LIBERI_CLASS_LIST(DECL_CLASS)
+#undef DECL_CLASS
+
+#define DEF_CLASS(id, a, b, cc, d)                                                         \
+  int launch_class_##id(const ClassDev &cd, const BuildArgs &ba, hipStream_t st, int nwg) { \
+    return launch_one<a, b, cc, d>(cd, ba, st, nwg);                                        \
+  }                                                                                         \
+  int64_t scratch_class_##id() { return scratch_one<a, b, cc, d>(); }
+
+#ifndef LIBERI_CLASS_GROUP
+#define LIBERI_CLASS_GROUP -2
+#endif
+#if LIBERI_CLASS_GROUP == 0 || LIBERI_CLASS_GROUP == -2
+LIBERI_CLASS_LIST_G0(DEF_CLASS)
+#endif
+#if LIBERI_CLASS_GROUP == 1 || LIBERI_CLASS_GROUP == -2
+LIBERI_CLASS_LIST_G1(DEF_CLASS)
+#endif
+#if LIBERI_CLASS_GROUP == 2 || LIBERI_CLASS_GROUP == -2
+LIBERI_CLASS_LIST_G2(DEF_CLASS)
+#endif
+#if LIBERI_CLASS_GROUP == 3 || LIBERI_CLASS_GROUP == -2
+LIBERI_CLASS_LIST_G3(DEF_CLASS)
+#endif
+#if LIBERI_CLASS_GROUP == 4 || LIBERI_CLASS_GROUP == -2
+LIBERI_CLASS_LIST_G4(DEF_CLASS)
+#endif
+#if LIBERI_CLASS_GROUP == 5 || LIBERI_CLASS_GROUP == -2
+LIBERI_CLASS_LIST_G5(DEF_CLASS)
+#endif
+#if LIBERI_CLASS_GROUP == 6 || LIBERI_CLASS_GROUP == -2
+LIBERI_CLASS_LIST_G6(DEF_CLASS)
+#endif
+#if LIBERI_CLASS_GROUP == 7 || LIBERI_CLASS_GROUP == -2
+LIBERI_CLASS_LIST_G7(DEF_CLASS)
+#endif
+#undef DEF_CLASS
+
+#if LIBERI_CLASS_GROUP < 0
+int launch_class(int c, const ClassDev &cd, const BuildArgs &ba, void *stream, int nwg) {
+  hipStream_t st = (hipStream_t)stream;
+  if (!(cd.xswitch <= class_rys_xmax(c))) return hipErrorInvalidValue;
+  switch (c) {
+#define X(id, a, b, cc, d) \
+  case id:                 \
+    return launch_class_##id(cd, ba, st, nwg);
+    LIBERI_CLASS_LIST(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+double class_rys_xmax(int c) {
+  switch (c) {
+#define X(id, a, b, cc, d) \
+  case id:                 \
+    return rys_xmax_host[(a + b + cc + d) / 2 + 1];
+    LIBERI_CLASS_LIST(X)
+#undef X
+    default:
+      return 0.0;
+  }
+}
+
+int class_nroots(int c) {
+  switch (c) {
+#define X(id, a, b, cc, d) \
+  case id:                 \
+    return (a + b + cc + d) / 2 + 1;
+    LIBERI_CLASS_LIST(X)
+#undef X
+    default:
+      return 0;
+  }
+}
+
+int64_t class_scratch_doubles(int c) {
+  switch (c) {
+#define X(id, a, b, cc, d) \
+  case id:                 \
+    return scratch_class_##id();
+    LIBERI_CLASS_LIST(X)
+#undef X
+    default:
+      return 0;
+  }
+}
+#endif
+
+}  // namespace liberi

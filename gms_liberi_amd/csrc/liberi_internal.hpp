@@ -1,0 +1,105 @@
+// Internal data model of the MI355X Fock build (host + device).
+//
+// The reference keeps, per pair class, arrays of primitive-pair data behind
+// CSR offsets (shell_pair_t, src/liberi_types.f90:80-92) and rebuilds the
+// screened pair lists on the host for every kernel call
+// (e.g. src/rhf/rot_axis/int0001.F90:65-84).  Here every shell pair of every
+// class lives once in HBM as a fixed-size record, every primitive pair as a
+// 64-byte record, and each of the 55 classes owns its screened, sorted pair
+// lists plus a segment table that enumerates the surviving quartets without
+// materialising them.
+#pragma once
+
+#include <cstdint>
+
+namespace liberi {
+
+constexpr int kNumClasses = 55;
+constexpr int kNumPairClasses = 10;
+constexpr double kRefCutoff = 5.00000000000000018e-11;  // src/liberi_parameters.F90:38
+constexpr double kTwoPi52 = 34.98683665524971863;        // 2 pi^(5/2), :20
+
+// pair classes in the order of shell_pair_container_t (src/liberi_types.f90:94-109)
+enum PairClass { P_SS, P_SP, P_PP, P_SD, P_PD, P_DD, P_SF, P_PF, P_DF, P_FF };
+constexpr int kPairL[kNumPairClasses][2] = {{0, 0}, {0, 1}, {1, 1}, {0, 2}, {1, 2},
+                                            {2, 2}, {0, 3}, {1, 3}, {2, 3}, {3, 3}};
+
+enum QuartetTest { CMP_GT, CMP_GE, CMP_NONE };
+enum PreScreen { PS_CROSS, PS_SELF };
+
+struct ClassDef {
+  const char *name;
+  int bra, ket;  // pair classes; bra == ket marks a diagonal class (kl <= ij)
+  int cmp, ps;
+};
+
+// Dispatch order of src/liberi_driver.F90:69-158.  Quartet test and pair
+// pre-screen per kernel file: the hand-written rotated-axis kernels normalise
+// each list's pre-screen by the OTHER list's largest Schwarz value
+// (int0001.F90:67,76), the generated Rys/ERIC kernels by their own
+// (int3300_rysgen.F90, int0030_ericgen.F90); int0000/int0101/int1111 skip when
+// x*x < cut (int0000.F90:107), int2222gen and int3333 have no quartet test
+// (int2222_rysgen.F90:428-431), every other kernel keeps x*x > cut.
+extern const ClassDef kClasses[kNumClasses];
+
+// One shell pair, A = the shell of higher angular momentum (A first for equal L).
+struct PairRec {
+  int32_t prim_off;  // first primitive pair record
+  int32_t nprim;
+  int32_t bfA, bfB;  // first basis function (0-based) of A and B
+  double ABx, ABy, ABz;
+  int32_t sameAB;    // A and B are one shell
+  int32_t shA;       // shell index of A (0-based)
+};
+
+// One primitive pair (a in A, b in B): p = a + b, P, P - A, and
+// K = c_a c_b exp(-ab/p |A-B|^2) / p.
+struct alignas(64) PrimRec {
+  double p, Px, Py, Pz, PAx, PAy, PAz, K;
+};
+
+// Device view of one class's quartet enumeration.  Kept pair lists are sorted
+// by (primitive count, Schwarz value descending); a segment is one kept bra
+// row against one primitive-count group of kets, and its surviving kets are a
+// prefix of that group (the quartet test is monotone in the ket's Schwarz
+// value).  Quartet q of the class lies in segment s with
+// seg_start[s] <= q < seg_start[s+1], ket = ket_list[seg_ket0[s] + q - seg_start[s]].
+struct ClassDev {
+  const int32_t *bra_list;  // pair indices (into the PairRec array)
+  const int32_t *ket_list;
+  const int64_t *seg_start;  // nseg + 1
+  const int32_t *seg_bra;    // bra pair index of the segment
+  const int32_t *seg_ket0;   // position in ket_list of the segment's first ket
+  int32_t nseg;
+  int32_t diag;
+  int64_t q_begin, q_end;    // this rank's share of the class's quartets
+  double xswitch;            // Rys argument from which the asymptotic rule is used
+};
+
+struct BuildArgs {
+  const PairRec *pairs;
+  const PrimRec *prims;
+  const double *D;
+  double *F;
+  double value_cut;
+  double *scratch;       // per-workgroup scratch for the cooperative kernels
+  int64_t scratch_per_wg;  // doubles
+};
+
+// Launch the kernel of class c over [q_begin, q_end).  Returns hipError_t.
+int launch_class(int c, const ClassDev &cd, const BuildArgs &ba, void *stream, int num_wg_coop);
+// doubles of scratch the cooperative kernel of class c needs per workgroup (0 if none)
+int64_t class_scratch_doubles(int c);
+// Rys roots of class c and the end of its tabulated range
+int class_nroots(int c);
+double class_rys_xmax(int c);
+// workgroups the cooperative kernels use
+constexpr int kCoopWorkgroups = 2048;
+
+// Segment lengths on the device: one thread per (kept bra row, ket group).
+int launch_segment_lengths(const double *bra_x, const int32_t *bra_group, const int32_t *bra_pos_in_group,
+                           int32_t nbra, const double *ket_x, const int32_t *group_begin,
+                           const int32_t *group_end, int32_t ngroup, int32_t diag, int32_t cmp,
+                           double cut, int64_t *seg_len, void *stream);
+
+}  // namespace liberi

@@ -1,0 +1,119 @@
+/*
+ * liberi_hip -- MI355X-native Schwarz-screened RHF Fock build.
+ *
+ * C ABI of the drop-in replacement for gms_libERI's handle API
+ * (src/interface/liberi_interface.F90) and its GAMESS direct-link wrapper
+ * (src/interface/gms_liberi_wrapper.F90).  Plain pointers and sizes only.
+ * Every entry point returns LIBERI_OK (0) or a LIBERI_E* status and records a
+ * message readable with liberi_c_last_error() (thread-local).
+ *
+ * Array conventions are the reference's: 1-based sh_loc / atom_num /
+ * atom_loc, ang_mom 1..4 = s..f, coordinates in bohr, density and Fock packed
+ * lower-triangular with idx(i,j) = i(i-1)/2 + j (1-based, i >= j), density
+ * not doubled off the diagonal, Fock overwritten with S o (J[D] - K[D]/2)
+ * (1 on the diagonal of S, 2 off it).
+ */
+#ifndef LIBERI_HIP_H
+#define LIBERI_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LIBERI_OK 0
+#define LIBERI_EARG 1    /* bad argument (NULL, negative size, inconsistent basis) */
+#define LIBERI_ESTATE 2  /* call out of order, e.g. fock build before setup */
+#define LIBERI_ESIZE 3   /* matrix size n != num_bas(num_bas+1)/2 */
+#define LIBERI_EDEVICE 4 /* HIP runtime failure or no usable gfx950 device */
+#define LIBERI_ENOMEM 5  /* device or host allocation failed */
+
+#define LIBERI_NCLASS 55 /* integral classes, dispatch order of src/liberi_driver.F90:69-158 */
+
+typedef struct liberi_ctx liberi_ctx;
+
+/* Replaces liberi_create (src/interface/liberi_interface.F90:46-55). */
+int liberi_c_create(liberi_ctx **ctx);
+
+/* Replaces liberi_destroy (src/interface/liberi_interface.F90:57-68): cleans
+ * up if set up, frees the handle, sets *ctx = NULL. */
+int liberi_c_destroy(liberi_ctx **ctx);
+
+/* Replaces liberi_setup (src/interface/liberi_interface.F90:72-200): deep-copies
+ * the basis, builds the shell-pair structure-of-arrays and the screened
+ * quartet lists on the device.  Calling it on a set-up handle cleans up first
+ * (:133-136).  Arrays: ang_mom..end_bas have nsh entries; exponents and
+ * contr_coef_* mxgtot*nsh; coords 3*natoms; schwrz_int nsh(nsh+1)/2 (packed by
+ * shell pair like the density).  my_rank/num_procs select this rank's share
+ * of every class, as the reference's rank split does. */
+int liberi_c_setup(liberi_ctx *ctx, int32_t nsh, int32_t natoms, int32_t num_bas, int32_t mxgtot,
+                   const int32_t *ang_mom, const int32_t *contr_num, const int32_t *sh_loc,
+                   const int32_t *atom_num, const int32_t *atom_loc, const int32_t *start_bas,
+                   const int32_t *end_bas, const double *exponents, const double *contr_coef_s,
+                   const double *contr_coef_p, const double *contr_coef_d, const double *contr_coef_f,
+                   const double *coords, const double *schwrz_int, int32_t my_rank, int32_t num_procs);
+
+/* Replaces liberi_fock_build (src/interface/liberi_interface.F90:202-227) and
+ * through it compute_integrals (src/liberi_driver.F90:52-167).  Host arrays of
+ * n = num_bas(num_bas+1)/2 doubles; fock is overwritten.  Blocks until done. */
+int liberi_c_fock_build(liberi_ctx *ctx, const double *density, double *fock, int64_t n);
+
+/* Same build on device-resident arrays (hipMalloc'd, n doubles each) on the
+ * given hipStream_t (NULL = the handle's stream).  Asynchronous: returns after
+ * enqueueing.  No counterpart in the reference (which maps D/F per call,
+ * src/liberi_driver.F90:66,163); for callers that keep D and F in HBM. */
+int liberi_c_fock_build_device(liberi_ctx *ctx, const double *d_density, double *d_fock, int64_t n,
+                               void *stream);
+
+/* Replaces liberi_cleanup (src/interface/liberi_interface.F90:229-260):
+ * releases the basis and all device data; the handle can be set up again. */
+int liberi_c_cleanup(liberi_ctx *ctx);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *liberi_c_last_error(void);
+
+/* Runtime knobs (compile-time parameters in the reference,
+ * src/liberi_parameters.F90:38).  Take effect at the next setup. */
+#define LIBERI_OPT_SCHWARZ_CUTOFF 1 /* quartet screening cutoff, default 5.00000000000000018e-11 */
+#define LIBERI_OPT_VALUE_CUTOFF 2   /* |integral| skip threshold, default 5.00000000000000018e-11 */
+#define LIBERI_OPT_FP32_TAU 3       /* mixed precision switch tau; 0 = plain FP64 (default) */
+#define LIBERI_OPT_EXACT_LARGE_T 4  /* 0 (default): large-argument forms from the reference's switch
+                                       points (T > 25 for its Boys-based classes, X >= 37..65 by root
+                                       count for its Rys classes); 1: only where exact to double */
+int liberi_c_set_option(liberi_ctx *ctx, int32_t option, double value);
+int liberi_c_get_option(liberi_ctx *ctx, int32_t option, double *value);
+
+/* Statistics of the handle (no counterpart in the reference, which computes
+ * the surviving quartet counts implicitly and reports none). */
+typedef struct {
+  int64_t quartets[LIBERI_NCLASS];      /* surviving contracted shell quartets per class (this rank) */
+  int64_t prim_quartets[LIBERI_NCLASS]; /* primitive quartets of those */
+  int64_t total_quartets;
+  int64_t fp32_quartets;                /* evaluated in FP32 in the last build */
+  double last_build_ms;                 /* device time of the last build */
+  int64_t device_bytes;                 /* HBM held by the handle */
+} liberi_stats;
+int liberi_c_get_stats(liberi_ctx *ctx, liberi_stats *out);
+
+/* Name of class c (reference kernel name, e.g. "int0000"). */
+const char *liberi_c_class_name(int32_t c);
+
+/* GAMESS direct-link entry points with one process-wide handle: replace
+ * liberi_init / liberi_fock / liberi_finalize of
+ * src/interface/gms_liberi_wrapper.F90:29-146.  Note the argument order of
+ * liberi_c_init (natoms, num_bas, nsh, mxgtot, ...) as in :29.  liberi_c_fock
+ * returns LIBERI_ESTATE before init and LIBERI_ESIZE for a wrong n (:117-127). */
+int liberi_c_init(int32_t natoms, int32_t num_bas, int32_t nsh, int32_t mxgtot, const int32_t *ang_mom,
+                  const int32_t *contr_num, const int32_t *sh_loc, const int32_t *atom_num,
+                  const int32_t *atom_loc, const int32_t *start_bas, const int32_t *end_bas,
+                  const double *exponents, const double *contr_coef_s, const double *contr_coef_p,
+                  const double *contr_coef_d, const double *contr_coef_f, const double *coords,
+                  const double *schwrz_int, int32_t my_rank, int32_t num_procs);
+int liberi_c_fock(const double *density, double *fock, int64_t n);
+int liberi_c_finalize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
