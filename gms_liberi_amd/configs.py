@@ -160,6 +160,13 @@ def c60(seed: int = 13) -> Case:
     return build_case("c60", ["C"] * 60, truncated_icosahedron(3.55), _heavy, 180, seed)
 
 
+def c60_fragment(natoms: int = 12, seed: int = 13) -> Case:
+    """The `natoms` cage atoms nearest atom 0, with the C60 basis recipe."""
+    pts = truncated_icosahedron(3.55)
+    idx = np.argsort(np.linalg.norm(pts - pts[0], axis=1))[:natoms]
+    return build_case(f"c{natoms}", ["C"] * natoms, pts[idx], _heavy, 3 * natoms, seed)
+
+
 def cn_chain(seed: int = 14, natoms: int = 20) -> Case:
     bond, ang = 1.45, math.radians(112.0)
     dx, dy = bond * math.sin(ang / 2), bond * math.cos(ang / 2)
@@ -185,7 +192,7 @@ def cluster(natoms: int, seed: int) -> Case:
     return build_case(f"cluster{natoms}_{seed}", syms, pts, _heavy, max(1, nelec // 2), seed + 1000)
 
 
-GENERATORS = {"water": water, "ethane": ethane, "c60": c60, "cn_chain": cn_chain}
+GENERATORS = {"water": water, "ethane": ethane, "c60": c60, "cn_chain": cn_chain, "c12": c60_fragment}
 
 
 def make_case(name: str, schwarz=None) -> Case:

@@ -97,7 +97,30 @@ struct liberi_ctx {
   int64_t scratch_per_wg = 0;
   ClassState cls[kNumClasses];
   liberi_stats stats{};
+  // diagnostics
+  bool class_timing = false, timing_pending = false, build_pending = false;
+  int only_class = -1;
+  hipEvent_t cev0[kNumClasses] = {}, cev1[kNumClasses] = {};
 };
+
+// fold the events of the last build into the statistics (waits for it)
+static void collect_timing(liberi_ctx *h) {
+  if (h->build_pending) {
+    if (hipEventSynchronize(h->ev1) == hipSuccess) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, h->ev0, h->ev1) == hipSuccess) h->stats.last_build_ms = ms;
+    }
+    h->build_pending = false;
+  }
+  if (!h->timing_pending) return;
+  for (int c = 0; c < kNumClasses; ++c) {
+    h->stats.class_ms[c] = 0.0;
+    if (!h->cls[c].active || !h->cev0[c]) continue;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, h->cev0[c], h->cev1[c]) == hipSuccess) h->stats.class_ms[c] = ms;
+  }
+  h->timing_pending = false;
+}
 
 static int dev_alloc(liberi_ctx *h, DevBuf &b, size_t bytes) {
   b.bytes = bytes;
@@ -187,6 +210,10 @@ int liberi_c_destroy(liberi_ctx **ctx) {
   liberi_c_cleanup(h);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
+  for (int c = 0; c < kNumClasses; ++c) {
+    if (h->cev0[c]) (void)hipEventDestroy(h->cev0[c]);
+    if (h->cev1[c]) (void)hipEventDestroy(h->cev1[c]);
+  }
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   *ctx = nullptr;
@@ -195,8 +222,15 @@ int liberi_c_destroy(liberi_ctx **ctx) {
 
 int liberi_c_set_option(liberi_ctx *h, int32_t opt, double v) {
   if (!h) return fail(LIBERI_EARG, "liberi_c_set_option: NULL handle");
+  if (opt == LIBERI_OPT_ONLY_CLASS) {
+    if (!(v >= -1.0 && v < kNumClasses) || v != std::floor(v))
+      return fail(LIBERI_EARG, "liberi_c_set_option: ONLY_CLASS must be -1 or a class index 0..54");
+    h->only_class = (int)v;
+    return LIBERI_OK;
+  }
   if (!(v >= 0.0) || !std::isfinite(v)) return fail(LIBERI_EARG, "liberi_c_set_option: value must be finite, >= 0");
   switch (opt) {
+    case LIBERI_OPT_CLASS_TIMING: h->class_timing = v != 0.0; break;
     case LIBERI_OPT_SCHWARZ_CUTOFF: h->schwarz_cut = v; break;
     case LIBERI_OPT_VALUE_CUTOFF: h->value_cut = v; break;
     case LIBERI_OPT_FP32_TAU: h->fp32_tau = v; break;
@@ -213,6 +247,8 @@ int liberi_c_get_option(liberi_ctx *h, int32_t opt, double *v) {
     case LIBERI_OPT_VALUE_CUTOFF: *v = h->value_cut; break;
     case LIBERI_OPT_FP32_TAU: *v = h->fp32_tau; break;
     case LIBERI_OPT_EXACT_LARGE_T: *v = h->exact_large_t; break;
+    case LIBERI_OPT_CLASS_TIMING: *v = h->class_timing; break;
+    case LIBERI_OPT_ONLY_CLASS: *v = h->only_class; break;
     default: return fail(LIBERI_EARG, "liberi_c_get_option: unknown option " + std::to_string(opt));
   }
   return LIBERI_OK;
@@ -220,6 +256,7 @@ int liberi_c_get_option(liberi_ctx *h, int32_t opt, double *v) {
 
 int liberi_c_get_stats(liberi_ctx *h, liberi_stats *out) {
   if (!h || !out) return fail(LIBERI_EARG, "liberi_c_get_stats: NULL argument");
+  collect_timing(h);
   *out = h->stats;
   return LIBERI_OK;
 }
@@ -494,11 +531,22 @@ static int build_on_device(liberi_ctx *h, const double *dD, double *dF, hipStrea
   HIP_TRY(hipEventRecord(h->ev0, st));
   for (int c = 0; c < kNumClasses; ++c) {
     if (!h->cls[c].active) continue;
+    if (h->only_class >= 0 && c != h->only_class) continue;
+    if (h->class_timing) {
+      if (!h->cev0[c]) HIP_TRY(hipEventCreate(&h->cev0[c]));
+      if (!h->cev1[c]) HIP_TRY(hipEventCreate(&h->cev1[c]));
+      HIP_TRY(hipEventRecord(h->cev0[c], st));
+    }
     int e = launch_class(c, h->cls[c].dev, ba, st, kCoopWorkgroups);
     if (e != hipSuccess)
       return fail(LIBERI_EDEVICE, std::string("launch ") + kClasses[c].name + ": " + hipGetErrorString((hipError_t)e));
+    if (h->class_timing) HIP_TRY(hipEventRecord(h->cev1[c], st));
   }
   HIP_TRY(hipEventRecord(h->ev1, st));
+  h->build_pending = true;
+  h->timing_pending = h->class_timing;
+  if (!h->class_timing)
+    for (double &v : h->stats.class_ms) v = 0.0;
   return LIBERI_OK;
 }
 
@@ -515,8 +563,7 @@ int liberi_c_fock_build(liberi_ctx *h, const double *density, double *fock, int6
   if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(fock, h->fock.p, ntri * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
-  float ms = 0.f;
-  if (hipEventElapsedTime(&ms, h->ev0, h->ev1) == hipSuccess) h->stats.last_build_ms = ms;
+  collect_timing(h);
   return LIBERI_OK;
 }
 

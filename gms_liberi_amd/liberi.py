@@ -33,6 +33,8 @@ OPT_SCHWARZ_CUTOFF = 1
 OPT_VALUE_CUTOFF = 2
 OPT_FP32_TAU = 3
 OPT_EXACT_LARGE_T = 4
+OPT_CLASS_TIMING = 5
+OPT_ONLY_CLASS = 6
 
 _STATUS = {1: "EARG", 2: "ESTATE", 3: "ESIZE", 4: "EDEVICE", 5: "ENOMEM"}
 
@@ -46,7 +48,7 @@ class LiberiError(RuntimeError):
 class Stats(C.Structure):
     _fields_ = [("quartets", C.c_int64 * NCLASS), ("prim_quartets", C.c_int64 * NCLASS),
                 ("total_quartets", C.c_int64), ("fp32_quartets", C.c_int64), ("last_build_ms", C.c_double),
-                ("device_bytes", C.c_int64)]
+                ("device_bytes", C.c_int64), ("class_ms", C.c_double * NCLASS)]
 
 
 _lib = None
@@ -172,7 +174,8 @@ class Handle:
         _check(self._lib.liberi_c_get_stats(self._h, C.byref(s)))
         return {"quartets": np.array(s.quartets[:], np.int64), "prim_quartets": np.array(s.prim_quartets[:], np.int64),
                 "total_quartets": s.total_quartets, "fp32_quartets": s.fp32_quartets,
-                "last_build_ms": s.last_build_ms, "device_bytes": s.device_bytes}
+                "last_build_ms": s.last_build_ms, "device_bytes": s.device_bytes,
+                "class_ms": np.array(s.class_ms[:], np.float64)}
 
 
 # ---------------------------------------------- reference-named functions

@@ -81,6 +81,8 @@ const char *liberi_c_last_error(void);
 #define LIBERI_OPT_EXACT_LARGE_T 4  /* 0 (default): large-argument forms from the reference's switch
                                        points (T > 25 for its Boys-based classes, X >= 37..65 by root
                                        count for its Rys classes); 1: only where exact to double */
+#define LIBERI_OPT_CLASS_TIMING 5   /* 1: time every class launch with HIP events (stats.class_ms) */
+#define LIBERI_OPT_ONLY_CLASS 6     /* >= 0: build only this class (diagnostics); -1 (default): all */
 int liberi_c_set_option(liberi_ctx *ctx, int32_t option, double value);
 int liberi_c_get_option(liberi_ctx *ctx, int32_t option, double *value);
 
@@ -93,6 +95,8 @@ typedef struct {
   int64_t fp32_quartets;                /* evaluated in FP32 in the last build */
   double last_build_ms;                 /* device time of the last build */
   int64_t device_bytes;                 /* HBM held by the handle */
+  double class_ms[LIBERI_NCLASS];       /* per-class device time of the last build, when
+                                           LIBERI_OPT_CLASS_TIMING is on (else 0) */
 } liberi_stats;
 int liberi_c_get_stats(liberi_ctx *ctx, liberi_stats *out);
 

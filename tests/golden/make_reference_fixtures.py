@@ -171,11 +171,37 @@ def config_fixtures(names):
               f" quartets={int(counts.sum())}")
 
 
+FRAGMENT_CLASSES = ["int0022", "int0112", "int0122", "int1102", "int2220"]
+
+
+def fragment_fixture():
+    """12-atom piece of the C60 cage and the reference's per-class Fock
+    contributions (one kernel at a time, oracle/ref_classes.f90) for a few
+    rotated-axis classes and one Rys class."""
+    from gms_liberi_amd import configs
+    from tests.oracle_ctypes import Oracle
+
+    case = configs.make_case("c12", schwarz=Oracle().schwarz)
+    tmp = OUT / "work"
+    tmp.mkdir(parents=True, exist_ok=True)
+    cf, ff = tmp / "c12.case", tmp / "c12.fock"
+    case.write_stream(cf)
+    out = {}
+    for k in FRAGMENT_CLASSES:
+        run([str(OUT / "liberi_ref_class"), str(cf), str(ff), k], env=REF_ENV)
+        out[k] = np.fromfile(ff, "<f8")
+    case.source = ("gms_liberi_amd/configs.py:c60_fragment (12 cage atoms); per-class Fock of the reference build "
+                   "in frag_c12_ref_classes.npz")
+    case.save_npz(GOLD / "frag_c12.npz")
+    np.savez_compressed(GOLD / "frag_c12_ref_classes.npz", **out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-build", action="store_true")
     ap.add_argument("--configs", nargs="*", default=["water", "ethane"])
     ap.add_argument("--no-ref-tests", action="store_true")
+    ap.add_argument("--fragment", action="store_true")
     a = ap.parse_args()
     if not a.skip_build:
         build_reference()
@@ -183,6 +209,8 @@ def main():
         reference_test_fixtures()
     if a.configs:
         config_fixtures(a.configs)
+    if a.fragment:
+        fragment_fixture()
 
 
 if __name__ == "__main__":

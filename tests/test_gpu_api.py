@@ -1,0 +1,145 @@
+"""Handle lifecycle, error behaviour and rank splitting of the HIP library (GPU).
+
+Error behaviour follows the reference: a Fock build before setup is an error
+(src/interface/liberi_interface.F90:219-222); the GAMESS wrapper rejects a
+build before liberi_init and a wrong matrix size
+(src/interface/gms_liberi_wrapper.F90:117-127); setup on a set-up handle
+cleans up first (liberi_interface.F90:133-136).
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from gms_liberi_amd import liberi
+from gms_liberi_amd.caseio import Case
+from tests.test_oracle import screening_basis
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+@pytest.mark.gpu
+def test_build_before_setup_is_an_error():
+    h = liberi.Handle()
+    with pytest.raises(liberi.LiberiError) as e:
+        h.fock_build(np.zeros(10))
+    assert e.value.code == 2
+    h.destroy()
+
+
+@pytest.mark.gpu
+def test_wrong_size_and_resetup():
+    case = Case.load_npz(GOLD / "cfg_water.npz")
+    h = liberi.Handle()
+    h.setup(case.basis)
+    with pytest.raises(liberi.LiberiError) as e:
+        h.fock_build(np.zeros(case.basis.ntri + 1))
+    assert e.value.code == 3
+    f1 = h.fock_build(case.density)
+    h.setup(case.basis)  # implicit cleanup
+    f2 = h.fock_build(case.density)
+    assert np.max(np.abs(f1 - f2)) <= 1e-12
+    h.cleanup()
+    with pytest.raises(liberi.LiberiError):
+        h.fock_build(case.density)
+    h.destroy()
+
+
+@pytest.mark.gpu
+def test_gamess_wrapper_entry_points():
+    case = Case.load_npz(GOLD / "ref_h2o_ccpvdz.npz")
+    b = case.basis
+    f = np.zeros(b.ntri)
+    with pytest.raises(liberi.LiberiError) as e:
+        liberi.liberi_fock(case.density, f, b.ntri)
+    assert e.value.code == 2
+    liberi.liberi_init(b.natoms, b.num_bas, b.nsh, b.mxgtot, b.ang_mom, b.contr_num, b.sh_loc, b.atom_num,
+                       b.atom_loc, b.start_bas, b.end_bas, b.exponents, b.coef_s, b.coef_p, b.coef_d, b.coef_f,
+                       b.coords, b.schwrz, 0, 1)
+    with pytest.raises(liberi.LiberiError) as e:
+        liberi.liberi_fock(case.density, f, b.ntri - 1)
+    assert e.value.code == 3
+    liberi.liberi_fock(case.density, f, b.ntri)
+    assert np.max(np.abs(f - case.fock)) <= 1e-10
+    liberi.liberi_finalize()
+
+
+@pytest.mark.gpu
+def test_reference_named_functions():
+    case = Case.load_npz(GOLD / "ref_h2_ccpvdz.npz")
+    b = case.basis
+    h = liberi.liberi_create()
+    liberi.liberi_setup(h, b.nsh, b.natoms, b.num_bas, b.mxgtot, b.ang_mom, b.contr_num, b.sh_loc, b.atom_num,
+                        b.atom_loc, b.start_bas, b.end_bas, b.exponents, b.coef_s, b.coef_p, b.coef_d, b.coef_f,
+                        b.coords, b.schwrz, 0, 1)
+    f = np.full(b.ntri, 7.0)  # overwritten, not accumulated (src/liberi_driver.F90:64)
+    liberi.liberi_fock_build(h, case.density, f, b.ntri)
+    assert np.max(np.abs(f - case.fock)) <= 1e-10
+    liberi.liberi_cleanup(h)
+    liberi.liberi_destroy(h)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nprocs", [2, 3])
+def test_rank_partials_sum_to_full(nprocs):
+    """Each rank returns its partial F; the caller's sum is the full F."""
+    case = Case.load_npz(GOLD / "cfg_ethane.npz")
+    h = liberi.Handle()
+    h.setup(case.basis)
+    full = h.fock_build(case.density)
+    total_q = h.stats()["quartets"].copy()
+    acc = np.zeros_like(full)
+    q = np.zeros_like(total_q)
+    for r in range(nprocs):
+        h.setup(case.basis, my_rank=r, num_procs=nprocs)
+        acc += h.fock_build(case.density)
+        q += h.stats()["quartets"]
+    h.destroy()
+    assert np.array_equal(q, total_q)
+    assert np.max(np.abs(acc - full)) <= 1e-12
+
+
+@pytest.mark.gpu
+def test_screening_counts_reference_targets():
+    d = json.loads((GOLD / "ref_screening_w2_631gd.json").read_text())
+    h = liberi.Handle()
+    h.setup(screening_basis(d))
+    counts = h.stats()["quartets"]
+    h.destroy()
+    names = liberi.class_names()
+    assert [int(counts[names.index(k)]) for k in d["classes"]] == d["alternative_target"]
+
+
+@pytest.mark.gpu
+def test_device_resident_build_matches_host_build():
+    from gms_liberi_amd import hipmem
+
+    case = Case.load_npz(GOLD / "cfg_ethane.npz")
+    h = liberi.Handle()
+    h.setup(case.basis)
+    host = h.fock_build(case.density)
+    D = hipmem.DeviceArray.from_host(case.density)
+    F = hipmem.DeviceArray.from_host(np.full(case.basis.ntri, 3.0))
+    s = hipmem.Stream()
+    h.fock_build_device(D.address, F.address, case.basis.ntri, s.address)
+    s.synchronize()
+    got = F.to_host()
+    h.destroy()
+    s.destroy()
+    assert np.max(np.abs(got - host)) <= 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["ref_h2o_ccpvtz.npz", "cfg_ethane.npz"])
+def test_exact_large_t_mode_matches_oracle(name, oracle):
+    """With the large-argument switch moved to where it is exact, the build
+    agrees with the independent McMurchie-Davidson oracle."""
+    case = Case.load_npz(GOLD / name)
+    h = liberi.Handle()
+    h.set_option(liberi.OPT_EXACT_LARGE_T, 1)
+    h.setup(case.basis)
+    f = h.fock_build(case.density)
+    h.destroy()
+    fo, _, _ = oracle.fock(case.basis, case.density)
+    assert np.max(np.abs(f - fo)) <= 1e-11 * max(1.0, np.max(np.abs(fo)))
