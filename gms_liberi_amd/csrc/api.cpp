@@ -86,6 +86,7 @@ struct ClassState {
 struct liberi_ctx {
   // options
   double schwarz_cut = kRefCutoff, value_cut = kRefCutoff, fp32_tau = 0.0, exact_large_t = 0.0;
+  double prim_cut = 1e-24;
   // basis
   bool initialized = false;
   int32_t nsh = 0, natoms = 0, num_bas = 0, mxgtot = 0, rank = 0, nprocs = 1;
@@ -235,6 +236,7 @@ int liberi_c_set_option(liberi_ctx *h, int32_t opt, double v) {
     case LIBERI_OPT_VALUE_CUTOFF: h->value_cut = v; break;
     case LIBERI_OPT_FP32_TAU: h->fp32_tau = v; break;
     case LIBERI_OPT_EXACT_LARGE_T: h->exact_large_t = v != 0.0; break;
+    case LIBERI_OPT_PRIM_CUTOFF: h->prim_cut = v; break;
     default: return fail(LIBERI_EARG, "liberi_c_set_option: unknown option " + std::to_string(opt));
   }
   return LIBERI_OK;
@@ -247,6 +249,7 @@ int liberi_c_get_option(liberi_ctx *h, int32_t opt, double *v) {
     case LIBERI_OPT_VALUE_CUTOFF: *v = h->value_cut; break;
     case LIBERI_OPT_FP32_TAU: *v = h->fp32_tau; break;
     case LIBERI_OPT_EXACT_LARGE_T: *v = h->exact_large_t; break;
+    case LIBERI_OPT_PRIM_CUTOFF: *v = h->prim_cut; break;
     case LIBERI_OPT_CLASS_TIMING: *v = h->class_timing; break;
     case LIBERI_OPT_ONLY_CLASS: *v = h->only_class; break;
     default: return fail(LIBERI_EARG, "liberi_c_get_option: unknown option " + std::to_string(opt));
@@ -377,6 +380,45 @@ int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, 
   }
   if (prims.size() > (size_t)INT32_MAX || pairs.size() > (size_t)INT32_MAX)
     return fail(LIBERI_EARG, "liberi_c_setup: basis too large for 32-bit pair indices");
+
+  // ---- primitive-pair screen.  Bound of one primitive pair's share of any
+  // integral: B = 2pi^(5/2)^(1/2) |K| (1 + |P-A| + 1/sqrt(p))^(la+lb) (the
+  // Rys 2-D factors are polynomials in P-A, P-Q and 1/2p of that degree, the
+  // Boys values are <= 1).  A primitive quartet contributes at most
+  // B_ab * B_cd; pairs with B_ab * max(B) < prim_cut change no integral by
+  // more than prim_cut per primitive quartet, far below the value cutoff, and
+  // are dropped.  Quartet lists and counts are untouched (a pair may end up
+  // with no primitives and then contributes nothing).
+  if (h->prim_cut > 0.0) {
+    std::vector<double> bound(prims.size());
+    double bmax = 0.0;
+    std::vector<int> pair_lsum(pairs.size());
+    for (int pc = 0; pc < kNumPairClasses; ++pc)
+      for (int32_t p : class_pairs[pc]) pair_lsum[p] = kPairL[pc][0] + kPairL[pc][1];
+    for (size_t ip = 0; ip < pairs.size(); ++ip) {
+      const PairRec &r = pairs[ip];
+      for (int k = 0; k < r.nprim; ++k) {
+        const PrimRec &q = prims[r.prim_off + k];
+        const double pa = std::sqrt(q.PAx * q.PAx + q.PAy * q.PAy + q.PAz * q.PAz);
+        const double b = std::sqrt(kTwoPi52) * std::fabs(q.K) * std::pow(1.0 + pa + 1.0 / std::sqrt(q.p), pair_lsum[ip]);
+        bound[r.prim_off + k] = b;
+        bmax = std::max(bmax, b);
+      }
+    }
+    std::vector<PrimRec> kept;
+    kept.reserve(prims.size());
+    for (size_t ip = 0; ip < pairs.size(); ++ip) {
+      PairRec &r = pairs[ip];
+      const int32_t off = (int32_t)kept.size();
+      for (int k = 0; k < r.nprim; ++k)
+        if (bound[r.prim_off + k] * bmax >= h->prim_cut) kept.push_back(prims[r.prim_off + k]);
+      r.prim_off = off;
+      r.nprim = (int32_t)kept.size() - off;
+    }
+    h->stats.prim_pairs_dropped = (int64_t)(prims.size() - kept.size());
+    prims.swap(kept);
+  }
+  h->stats.prim_pairs = (int64_t)prims.size();
 
   int rc;
   if ((rc = dev_upload(h, h->pairs, pairs))) return rc;

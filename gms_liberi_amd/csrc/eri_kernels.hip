@@ -788,6 +788,299 @@ __global__ __launch_bounds__((Coop<LA, LB, LC, LD>::NT)) void eri_coop_kernel(Cl
   }
 }
 
+// =================================================== group-per-quartet path
+// G lanes own one quartet; a one-wave workgroup holds 64/G quartets side by
+// side, each with its own LDS slot: 2-D Rys tables of the current primitive
+// quartet, two HRR buffers, the six density blocks.  Loop bounds are uniform
+// over the wave (primitive loop to the wave's longest quartet, others
+// predicated), so the single-wave barriers are cheap and always matched.
+template <int LA, int LB, int LC, int LD>
+struct GroupSize {
+  static constexpr int NEF = nrange(LA, LA + LB) * nrange(LC, LC + LD);
+  static constexpr int NR = (LA + LB + LC + LD) / 2 + 1;
+  static constexpr int raw = NEF / 8 <= 16 ? 16 : (NEF / 8 <= 32 ? 32 : 64);
+  static constexpr int G = (3 * NR > raw) ? 64 : raw;
+};
+
+template <int LA, int LB, int LC, int LD>
+struct Grp {
+  static constexpr int G = GroupSize<LA, LB, LC, LD>::G;
+  static constexpr int NQ = 64 / G;
+  static constexpr int LAB = LA + LB, LCD = LC + LD;
+  static constexpr int NR = (LAB + LCD) / 2 + 1;
+  static constexpr int NI = LAB + 1, NK = LCD + 1;
+  static constexpr int NE = nrange(LA, LAB), NF = nrange(LC, LCD);
+  static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  static constexpr int NABCD = NA * NB * NC * ND;
+  static constexpr int GS = 3 * NR * NI * NK;
+  static constexpr int BUF = hrr_max_buffer(LA, LB, LC, LD);
+  static constexpr int NAB = NA * NB, NCD = NC * ND, NAC = NA * NC, NBD = NB * ND, NAD = NA * ND, NBC = NB * NC;
+  static constexpr int DB = NAB + NCD + NAC + NBD + NAD + NBC;
+  static constexpr int SLOT = GS + 2 * BUF + DB;
+  static constexpr int NKC = (NE * NF + G - 1) / G;
+  static constexpr bool fits = SLOT * NQ * 8 <= 64 * 1024;
+};
+
+__device__ __forceinline__ int gpos_of(int L, int k) {
+  int x, y, z;
+  cart_decode(L, k, x, y, z);
+  return gamess_pos(x, y, z);
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba) {
+  using S = Grp<LA, LB, LC, LD>;
+  constexpr int G = S::G, NQ = S::NQ, NR = S::NR, NI = S::NI, NK = S::NK, NE = S::NE, NF = S::NF, NKC = S::NKC;
+  constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND, NABCD = S::NABCD;
+  constexpr int GSZ = NI * NK;
+  __shared__ double lds[NQ * S::SLOT];
+  __shared__ int s_npq[NQ];
+
+  const int lane = threadIdx.x, slot = lane / G, gl = lane - slot * G;
+  double *Gt = lds + slot * S::SLOT;
+  double *buf0 = Gt + S::GS, *buf1 = buf0 + S::BUF, *Db = buf1 + S::BUF;
+  double *DAB = Db, *DCD = DAB + S::NAB, *DAC = DCD + S::NCD, *DBD = DAC + S::NAC, *DAD = DBD + S::NBD,
+         *DBC = DAD + S::NAD;
+
+  int offx[NKC], offy[NKC], offz[NKC];
+#pragma unroll
+  for (int k = 0; k < NKC; ++k) {
+    const int c = gl + G * k;
+    if (c < NE * NF) {
+      int Le, ex, ey, ez, Lf, fx, fy, fz;
+      range_decode(LA, c / NF, Le, ex, ey, ez);
+      range_decode(LC, c % NF, Lf, fx, fy, fz);
+      offx[k] = ex * NK + fx;
+      offy[k] = ey * NK + fy;
+      offz[k] = ez * NK + fz;
+    } else {
+      offx[k] = offy[k] = offz[k] = 0;
+    }
+  }
+
+  const int64_t nq = cd.q_end - cd.q_begin;
+  for (int64_t base = (int64_t)blockIdx.x * NQ; base < nq; base += (int64_t)gridDim.x * NQ) {
+    const int64_t q = cd.q_begin + base + slot;
+    const bool valid = base + slot < nq;
+    int32_t ib = 0, ik = 0;
+    PairRec bra{}, ket{};
+    if (valid) {
+      decode_quartet(cd, q, ib, ik);
+      bra = ba.pairs[ib];
+      ket = ba.pairs[ik];
+    }
+    const int nk = valid ? ket.nprim : 0;
+    const int npq = valid ? bra.nprim * ket.nprim : 0;
+    if (gl == 0) s_npq[slot] = npq;
+    __syncthreads();
+    int maxpq = 0;
+#pragma unroll
+    for (int s = 0; s < NQ; ++s) maxpq = s_npq[s] > maxpq ? s_npq[s] : maxpq;
+
+    double acc[NKC];
+#pragma unroll
+    for (int k = 0; k < NKC; ++k) acc[k] = 0.0;
+
+    for (int t = 0; t < maxpq; ++t) {
+      const bool active = t < npq;
+      if (active && gl < 3 * NR) {
+        const int ip = t / nk, iq = t - ip * nk;
+        const PrimRec P = ba.prims[bra.prim_off + ip];
+        const PrimRec Q = ba.prims[ket.prim_off + iq];
+        const int r = gl / 3, dim = gl - 3 * (gl / 3);
+        const double pq = P.p + Q.p, ipq = 1.0 / pq;
+        const double pqx = P.Px - Q.Px, pqy = P.Py - Q.Py, pqz = P.Pz - Q.Pz;
+        const double X = P.p * Q.p * ipq * (pqx * pqx + pqy * pqy + pqz * pqz);
+        double t2, wr;
+        rys_root<NR>(X, cd.xswitch, r, t2, wr);
+        const double hp = 0.5 / P.p, hq = 0.5 / Q.p;
+        const double b00 = 0.5 * t2 * ipq;
+        const double b10 = hp - hp * Q.p * t2 * ipq;
+        const double b01 = hq - hq * P.p * t2 * ipq;
+        const double fa = Q.p * ipq * t2, fc = P.p * ipq * t2;
+        const double pqd = dim == 0 ? pqx : (dim == 1 ? pqy : pqz);
+        const double pa = dim == 0 ? P.PAx : (dim == 1 ? P.PAy : P.PAz);
+        const double qc = dim == 0 ? Q.PAx : (dim == 1 ? Q.PAy : Q.PAz);
+        const double g00 = dim == 2 ? wr * kTwoPi52 * P.K * Q.K * sqrt(ipq) : 1.0;
+        double g[NI][NK];
+        rys_2d<NI, NK>(g, pa - fa * pqd, qc + fc * pqd, b10, b01, b00, g00);
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+#pragma unroll
+          for (int k = 0; k < NK; ++k) Gt[(dim * NR + r) * GSZ + i * NK + k] = g[i][k];
+      }
+      __syncthreads();
+      if (active) {
+#pragma unroll
+        for (int k = 0; k < NKC; ++k) {
+          double v = 0.0;
+#pragma unroll
+          for (int r = 0; r < NR; ++r)
+            v += Gt[r * GSZ + offx[k]] * Gt[(NR + r) * GSZ + offy[k]] * Gt[(2 * NR + r) * GSZ + offz[k]];
+          acc[k] += v;
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < NKC; ++k) {
+      const int c = gl + G * k;
+      if (c < NE * NF) buf0[c] = acc[k];
+    }
+    __syncthreads();
+
+    // bra HRR: level j out[a in range(LA, LAB-j)][b in cart(j)][f]
+    double *in = buf0, *out = buf1;
+#pragma unroll
+    for (int j = 1; j <= LB; ++j) {
+      const int na = nrange(LA, S::LAB - j), nb = ncart(j), nbm = ncart(j - 1);
+      const int items = na * nb * NF;
+      for (int it = gl; it < items; it += G) {
+        const int f = it % NF, rest = it / NF, b = rest % nb, a = rest / nb;
+        int bx, by, bz, La, ax, ay, az;
+        cart_decode(j, b, bx, by, bz);
+        range_decode(LA, a, La, ax, ay, az);
+        const int dir = bx ? 0 : (by ? 1 : 2);
+        const int bmi = cidx(bx - (dir == 0), by - (dir == 1), bz - (dir == 2));
+        const int api = range_index(LA, ax + (dir == 0), ay + (dir == 1), az + (dir == 2));
+        const double abv = dir == 0 ? bra.ABx : (dir == 1 ? bra.ABy : bra.ABz);
+        out[(a * nb + b) * NF + f] = in[(api * nbm + bmi) * NF + f] + abv * in[(a * nbm + bmi) * NF + f];
+      }
+      __syncthreads();
+      double *tmp = in;
+      in = out;
+      out = tmp;
+    }
+#pragma unroll
+    for (int j = 1; j <= LD; ++j) {
+      const int nc = nrange(LC, S::LCD - j), nd = ncart(j), ndm = ncart(j - 1);
+      const int nin = nrange(LC, S::LCD - j + 1) * ndm, nout = nc * nd;
+      const int items = NA * NB * nout;
+      for (int it = gl; it < items; it += G) {
+        const int ab = it / nout, rest = it - ab * nout, c = rest / nd, d = rest - c * nd;
+        int dx, dy, dz, Lc, cx, cy, cz;
+        cart_decode(j, d, dx, dy, dz);
+        range_decode(LC, c, Lc, cx, cy, cz);
+        const int dir = dx ? 0 : (dy ? 1 : 2);
+        const int dmi = cidx(dx - (dir == 0), dy - (dir == 1), dz - (dir == 2));
+        const int cpi = range_index(LC, cx + (dir == 0), cy + (dir == 1), cz + (dir == 2));
+        const double cdv = dir == 0 ? ket.ABx : (dir == 1 ? ket.ABy : ket.ABz);
+        out[ab * nout + c * nd + d] = in[ab * nin + cpi * ndm + dmi] + cdv * in[ab * nin + c * ndm + dmi];
+      }
+      __syncthreads();
+      double *tmp = in;
+      in = out;
+      out = tmp;
+    }
+    // scale + value cutoff in place; gather the six density blocks
+    if (valid) {
+      for (int it = gl; it < NABCD; it += G) {
+        const int d = it % ND, c = (it / ND) % NC, b = (it / (ND * NC)) % NB, a = it / (ND * NC * NB);
+        int x0, y0, z0, x1, y1, z1, x2, y2, z2, x3, y3, z3;
+        cart_decode(LA, a, x0, y0, z0);
+        cart_decode(LB, b, x1, y1, z1);
+        cart_decode(LC, c, x2, y2, z2);
+        cart_decode(LD, d, x3, y3, z3);
+        const double v = in[it] * cart_scale(x0, y0, z0) * cart_scale(x1, y1, z1) * cart_scale(x2, y2, z2) *
+                         cart_scale(x3, y3, z3);
+        in[it] = fabs(v) < ba.value_cut ? 0.0 : v;
+      }
+      for (int it = gl; it < S::DB; it += G) {
+        int64_t i1, i2;
+        int k = it;
+        if (k < S::NAB) {
+          i1 = bra.bfA + gpos_of(LA, k / NB);
+          i2 = bra.bfB + gpos_of(LB, k % NB);
+        } else if ((k -= S::NAB) < S::NCD) {
+          i1 = ket.bfA + gpos_of(LC, k / ND);
+          i2 = ket.bfB + gpos_of(LD, k % ND);
+        } else if ((k -= S::NCD) < S::NAC) {
+          i1 = bra.bfA + gpos_of(LA, k / NC);
+          i2 = ket.bfA + gpos_of(LC, k % NC);
+        } else if ((k -= S::NAC) < S::NBD) {
+          i1 = bra.bfB + gpos_of(LB, k / ND);
+          i2 = ket.bfB + gpos_of(LD, k % ND);
+        } else if ((k -= S::NBD) < S::NAD) {
+          i1 = bra.bfA + gpos_of(LA, k / ND);
+          i2 = ket.bfB + gpos_of(LD, k % ND);
+        } else {
+          k -= S::NAD;
+          i1 = bra.bfB + gpos_of(LB, k / NC);
+          i2 = ket.bfA + gpos_of(LC, k % NC);
+        }
+        Db[it] = ba.D[tri(i1, i2)];
+      }
+    }
+    __syncthreads();
+    if (valid) {
+      double s = 1.0;
+      if (bra.sameAB) s *= 0.5;
+      if (ket.sameAB) s *= 0.5;
+      if (cd.diag && ib == ik) s *= 0.5;
+      const double *v = in;
+      double *F = ba.F;
+      for (int it = gl; it < S::DB; it += G) {
+        double acc2 = 0.0, fac;
+        int64_t fi, fj;
+        int k = it;
+        if (k < S::NAB) {  // F[ab] += 4s sum_cd v D[cd]
+          const int a = k / NB, b = k % NB;
+#pragma unroll
+          for (int e = 0; e < NC * ND; ++e) acc2 += v[(a * NB + b) * NC * ND + e] * DCD[e];
+          fi = bra.bfA + gpos_of(LA, a);
+          fj = bra.bfB + gpos_of(LB, b);
+          fac = 4.0 * s;
+        } else if ((k -= S::NAB) < S::NCD) {  // F[cd] += 4s sum_ab v D[ab]
+#pragma unroll
+          for (int e = 0; e < NA * NB; ++e) acc2 += v[e * NC * ND + k] * DAB[e];
+          fi = ket.bfA + gpos_of(LC, k / ND);
+          fj = ket.bfB + gpos_of(LD, k % ND);
+          fac = 4.0 * s;
+        } else if ((k -= S::NCD) < S::NAC) {  // F[ac] -= s sum_bd v D[bd]
+          const int a = k / NC, c = k % NC;
+#pragma unroll
+          for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) acc2 += v[((a * NB + b) * NC + c) * ND + d] * DBD[b * ND + d];
+          fi = bra.bfA + gpos_of(LA, a);
+          fj = ket.bfA + gpos_of(LC, c);
+          fac = -s;
+        } else if ((k -= S::NAC) < S::NBD) {  // F[bd] -= s sum_ac v D[ac]
+          const int b = k / ND, d = k % ND;
+#pragma unroll
+          for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) acc2 += v[((a * NB + b) * NC + c) * ND + d] * DAC[a * NC + c];
+          fi = bra.bfB + gpos_of(LB, b);
+          fj = ket.bfB + gpos_of(LD, d);
+          fac = -s;
+        } else if ((k -= S::NBD) < S::NAD) {  // F[ad] -= s sum_bc v D[bc]
+          const int a = k / ND, d = k % ND;
+#pragma unroll
+          for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) acc2 += v[((a * NB + b) * NC + c) * ND + d] * DBC[b * NC + c];
+          fi = bra.bfA + gpos_of(LA, a);
+          fj = ket.bfB + gpos_of(LD, d);
+          fac = -s;
+        } else {  // F[bc] -= s sum_ad v D[ad]
+          k -= S::NAD;
+          const int b = k / NC, c = k % NC;
+#pragma unroll
+          for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) acc2 += v[((a * NB + b) * NC + c) * ND + d] * DAD[a * ND + d];
+          fi = bra.bfB + gpos_of(LB, b);
+          fj = ket.bfA + gpos_of(LC, c);
+          fac = -s;
+        }
+        if (acc2 != 0.0) fock_add(F, fi, fj, fac * acc2);
+      }
+    }
+    __syncthreads();  // slots are reused by the next round
+  }
+}
+
 // ----------------------------------------------------- segment lengths
 #if !defined(LIBERI_CLASS_GROUP) || LIBERI_CLASS_GROUP < 0
 __global__ void segment_lengths_kernel(const double *bra_x, const int32_t *bra_group, const int32_t *bra_pos,
@@ -853,6 +1146,11 @@ static int launch_one(const ClassDev &cd, const BuildArgs &ba, hipStream_t st, i
     const int tpb = 256;
     hipLaunchKernelGGL((eri_small_kernel<LA, LB, LC, LD>), dim3((unsigned)((n + tpb - 1) / tpb)), dim3(tpb), 0,
                        st, cd, ba);
+  } else if constexpr (Grp<LA, LB, LC, LD>::fits) {
+    constexpr int NQ = Grp<LA, LB, LC, LD>::NQ;
+    const int64_t need = (n + NQ - 1) / NQ;
+    const int64_t g = need < kGroupWorkgroups ? need : kGroupWorkgroups;
+    hipLaunchKernelGGL((eri_group_kernel<LA, LB, LC, LD>), dim3((unsigned)g), dim3(64), 0, st, cd, ba);
   } else {
     using S = Coop<LA, LB, LC, LD>;
     const int64_t g = n < nwg ? n : nwg;
@@ -863,7 +1161,7 @@ static int launch_one(const ClassDev &cd, const BuildArgs &ba, hipStream_t st, i
 
 template <int LA, int LB, int LC, int LD>
 static int64_t scratch_one() {
-  if constexpr (use_small<LA, LB, LC, LD>())
+  if constexpr (use_small<LA, LB, LC, LD>() || Grp<LA, LB, LC, LD>::fits)
     return 0;
   else
     return Coop<LA, LB, LC, LD>::scratch();

@@ -95,6 +95,8 @@ int class_nroots(int c);
 double class_rys_xmax(int c);
 // workgroups the cooperative kernels use
 constexpr int kCoopWorkgroups = 2048;
+// one-wave workgroups the group-per-quartet kernels use at most (persistent loop)
+constexpr int kGroupWorkgroups = 32768;
 
 // Segment lengths on the device: one thread per (kept bra row, ket group).
 int launch_segment_lengths(const double *bra_x, const int32_t *bra_group, const int32_t *bra_pos_in_group,

@@ -35,6 +35,7 @@ OPT_FP32_TAU = 3
 OPT_EXACT_LARGE_T = 4
 OPT_CLASS_TIMING = 5
 OPT_ONLY_CLASS = 6
+OPT_PRIM_CUTOFF = 7
 
 _STATUS = {1: "EARG", 2: "ESTATE", 3: "ESIZE", 4: "EDEVICE", 5: "ENOMEM"}
 
@@ -48,7 +49,8 @@ class LiberiError(RuntimeError):
 class Stats(C.Structure):
     _fields_ = [("quartets", C.c_int64 * NCLASS), ("prim_quartets", C.c_int64 * NCLASS),
                 ("total_quartets", C.c_int64), ("fp32_quartets", C.c_int64), ("last_build_ms", C.c_double),
-                ("device_bytes", C.c_int64), ("class_ms", C.c_double * NCLASS)]
+                ("device_bytes", C.c_int64), ("class_ms", C.c_double * NCLASS), ("prim_pairs", C.c_int64),
+                ("prim_pairs_dropped", C.c_int64)]
 
 
 _lib = None
@@ -175,7 +177,8 @@ class Handle:
         return {"quartets": np.array(s.quartets[:], np.int64), "prim_quartets": np.array(s.prim_quartets[:], np.int64),
                 "total_quartets": s.total_quartets, "fp32_quartets": s.fp32_quartets,
                 "last_build_ms": s.last_build_ms, "device_bytes": s.device_bytes,
-                "class_ms": np.array(s.class_ms[:], np.float64)}
+                "class_ms": np.array(s.class_ms[:], np.float64), "prim_pairs": s.prim_pairs,
+                "prim_pairs_dropped": s.prim_pairs_dropped}
 
 
 # ---------------------------------------------- reference-named functions

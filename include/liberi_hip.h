@@ -83,6 +83,8 @@ const char *liberi_c_last_error(void);
                                        count for its Rys classes); 1: only where exact to double */
 #define LIBERI_OPT_CLASS_TIMING 5   /* 1: time every class launch with HIP events (stats.class_ms) */
 #define LIBERI_OPT_ONLY_CLASS 6     /* >= 0: build only this class (diagnostics); -1 (default): all */
+#define LIBERI_OPT_PRIM_CUTOFF 7    /* drop primitive pairs whose share of any integral is below this
+                                       bound (default 1e-24; 0 keeps every primitive pair) */
 int liberi_c_set_option(liberi_ctx *ctx, int32_t option, double value);
 int liberi_c_get_option(liberi_ctx *ctx, int32_t option, double *value);
 
@@ -97,6 +99,8 @@ typedef struct {
   int64_t device_bytes;                 /* HBM held by the handle */
   double class_ms[LIBERI_NCLASS];       /* per-class device time of the last build, when
                                            LIBERI_OPT_CLASS_TIMING is on (else 0) */
+  int64_t prim_pairs;                   /* primitive pairs kept on the device */
+  int64_t prim_pairs_dropped;           /* primitive pairs removed by LIBERI_OPT_PRIM_CUTOFF */
 } liberi_stats;
 int liberi_c_get_stats(liberi_ctx *ctx, liberi_stats *out);
 
