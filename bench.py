@@ -144,6 +144,23 @@ def cpu_leg(case, keep: float):
     return dict(seconds=t, quartets=int(q_sample), threads=threads, pair_fraction=frac)
 
 
+def copy_calibration(mib: int) -> None:
+    """One device-to-device copy of `mib` MiB: the reference point that turns
+    rocprof's FETCH_SIZE / WRITE_SIZE counters into HBM bytes."""
+    import ctypes
+
+    from gms_liberi_amd import hipmem
+
+    n = mib * 1024 * 1024 // 8
+    src, dst = hipmem.DeviceArray(n), hipmem.DeviceArray(n)
+    rt = hipmem.runtime()
+    for _ in range(2):
+        rc = rt.hipMemcpy(dst.ptr, src.ptr, ctypes.c_size_t(n * 8), 3)  # hipMemcpyDeviceToDevice
+        if rc != 0:
+            raise RuntimeError("hipMemcpy D2D failed")
+    log(f"copied {mib} MiB device to device twice")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,7 +169,12 @@ def main():
     ap.add_argument("--workload", default="c60", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-pairs", type=float, default=0.3, help="shell-pair fraction of the CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (rocprof counter calibration)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py: this build measures one GPU only (asked for {a.gpus}); "
               "the Fock build shards by quartet ranges with one all-reduce of F, which is not run here.",

@@ -180,6 +180,24 @@ __device__ __forceinline__ void rys_2d(double (&g)[NI][NK], double c00, double c
   }
 }
 
+// segment holding quartet q, searched in [lo, hi): seg_start[lo] <= q < seg_start[hi]
+__device__ __forceinline__ int find_segment(const ClassDev &cd, int64_t q, int lo, int hi) {
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (cd.seg_start[mid] <= q)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void quartet_of_segment(const ClassDev &cd, int s, int64_t q, int32_t &bra,
+                                                   int32_t &ket) {
+  bra = cd.seg_bra[s];
+  ket = cd.ket_list[cd.seg_ket0[s] + (int32_t)(q - cd.seg_start[s])];
+}
+
 // quartet index -> (bra pair, ket pair) through the class's segment table
 __device__ __forceinline__ void decode_quartet(const ClassDev &cd, int64_t q, int32_t &bra, int32_t &ket) {
   int lo = 0, hi = cd.nseg;  // seg_start[lo] <= q < seg_start[hi]
@@ -295,10 +313,18 @@ __global__ __launch_bounds__(256) void eri_small_kernel(ClassDev cd, BuildArgs b
   constexpr Range<LA, LA + LB> re{};
   constexpr Range<LC, LC + LD> rf{};
 
-  const int64_t q = cd.q_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // the block's quartets span segments [s_lo, s_hi]: two full searches, then
+  // every lane searches only that short range
+  __shared__ int s_lo, s_hi;
+  const int64_t q0 = cd.q_begin + (int64_t)blockIdx.x * blockDim.x;
+  const int64_t qlast = (q0 + blockDim.x < cd.q_end ? q0 + blockDim.x : cd.q_end) - 1;
+  if (threadIdx.x == 0) s_lo = find_segment(cd, q0, 0, cd.nseg);
+  if (threadIdx.x == 1) s_hi = find_segment(cd, qlast, 0, cd.nseg);
+  __syncthreads();
+  const int64_t q = q0 + threadIdx.x;
   if (q >= cd.q_end) return;
   int32_t ib, ik;
-  decode_quartet(cd, q, ib, ik);
+  quartet_of_segment(cd, find_segment(cd, q, s_lo, s_hi + 1), q, ib, ik);
   const PairRec bra = ba.pairs[ib];
   const PairRec ket = ba.pairs[ik];
 
