@@ -77,9 +77,11 @@ struct DevBuf {
 
 struct ClassState {
   bool active = false;
+  bool row_ok = false;  // a row-tiled kernel exists and fits in LDS
+  int variant = -1;     // digestion chosen for the class: 0 direct, 1 row-tiled (-1: not yet)
   int64_t nquart = 0, nprimq = 0;
   int32_t nseg = 0;
-  DevBuf ket_list, seg_start, seg_bra, seg_ket0;
+  DevBuf ket_list, seg_start, seg_bra, seg_ket0, tile_q, tile_seg;
   ClassDev dev{};
 };
 
@@ -87,6 +89,8 @@ struct liberi_ctx {
   // options
   double schwarz_cut = kRefCutoff, value_cut = kRefCutoff, fp32_tau = 0.0, exact_large_t = 0.0;
   double prim_cut = 1e-24;
+  int row_tiles = 1;  // 0 never, 1 per class (autotuned or default), 2 wherever they fit
+  bool autotune = true;
   // basis
   bool initialized = false;
   int32_t nsh = 0, natoms = 0, num_bas = 0, mxgtot = 0, rank = 0, nprocs = 1;
@@ -96,6 +100,12 @@ struct liberi_ctx {
   std::vector<DevBuf> owned;
   DevBuf pairs, prims, dens, fock, scratch;
   int64_t scratch_per_wg = 0;
+  DevBuf col_bf[4];     // compact columns per angular momentum -> basis function
+  int32_t ncol[4] = {};
+  DevBuf aux;           // per build: max |D|, FP32 quartet count
+  DevBuf tune_F;        // autotuner's scratch Fock matrix
+  hipEvent_t tev[2] = {};
+  bool fp32_pending = false;
   ClassState cls[kNumClasses];
   liberi_stats stats{};
   // diagnostics
@@ -112,6 +122,12 @@ static void collect_timing(liberi_ctx *h) {
       if (hipEventElapsedTime(&ms, h->ev0, h->ev1) == hipSuccess) h->stats.last_build_ms = ms;
     }
     h->build_pending = false;
+    if (h->fp32_pending) {
+      uint64_t n = 0;
+      if (hipMemcpy(&n, (uint64_t *)h->aux.p + 1, sizeof(n), hipMemcpyDeviceToHost) == hipSuccess)
+        h->stats.fp32_quartets = (int64_t)n;
+      h->fp32_pending = false;
+    }
   }
   if (!h->timing_pending) return;
   for (int c = 0; c < kNumClasses; ++c) {
@@ -157,6 +173,8 @@ static void release_device(liberi_ctx *h) {
     dev_free(c.seg_start);
     dev_free(c.seg_bra);
     dev_free(c.seg_ket0);
+    dev_free(c.tile_q);
+    dev_free(c.tile_seg);
     c = ClassState{};
   }
   dev_free(h->pairs);
@@ -164,6 +182,10 @@ static void release_device(liberi_ctx *h) {
   dev_free(h->dens);
   dev_free(h->fock);
   dev_free(h->scratch);
+  dev_free(h->aux);
+  dev_free(h->tune_F);
+  for (auto &b : h->col_bf) dev_free(b);
+  for (auto &n : h->ncol) n = 0;
   h->stats = liberi_stats{};
   h->initialized = false;
 }
@@ -211,6 +233,8 @@ int liberi_c_destroy(liberi_ctx **ctx) {
   liberi_c_cleanup(h);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
+  for (hipEvent_t e : h->tev)
+    if (e) (void)hipEventDestroy(e);
   for (int c = 0; c < kNumClasses; ++c) {
     if (h->cev0[c]) (void)hipEventDestroy(h->cev0[c]);
     if (h->cev1[c]) (void)hipEventDestroy(h->cev1[c]);
@@ -237,6 +261,14 @@ int liberi_c_set_option(liberi_ctx *h, int32_t opt, double v) {
     case LIBERI_OPT_FP32_TAU: h->fp32_tau = v; break;
     case LIBERI_OPT_EXACT_LARGE_T: h->exact_large_t = v != 0.0; break;
     case LIBERI_OPT_PRIM_CUTOFF: h->prim_cut = v; break;
+    case LIBERI_OPT_ROW_TILES:
+      if (v != 0.0 && v != 1.0 && v != 2.0) return fail(LIBERI_EARG, "liberi_c_set_option: ROW_TILES must be 0, 1 or 2");
+      h->row_tiles = (int)v;
+      break;
+    case LIBERI_OPT_AUTOTUNE:
+      h->autotune = v != 0.0;
+      for (auto &c : h->cls) c.variant = -1;  // decide again at the next build
+      break;
     default: return fail(LIBERI_EARG, "liberi_c_set_option: unknown option " + std::to_string(opt));
   }
   return LIBERI_OK;
@@ -252,6 +284,8 @@ int liberi_c_get_option(liberi_ctx *h, int32_t opt, double *v) {
     case LIBERI_OPT_PRIM_CUTOFF: *v = h->prim_cut; break;
     case LIBERI_OPT_CLASS_TIMING: *v = h->class_timing; break;
     case LIBERI_OPT_ONLY_CLASS: *v = h->only_class; break;
+    case LIBERI_OPT_ROW_TILES: *v = h->row_tiles; break;
+    case LIBERI_OPT_AUTOTUNE: *v = h->autotune; break;
     default: return fail(LIBERI_EARG, "liberi_c_get_option: unknown option " + std::to_string(opt));
   }
   return LIBERI_OK;
@@ -261,6 +295,11 @@ int liberi_c_get_stats(liberi_ctx *h, liberi_stats *out) {
   if (!h || !out) return fail(LIBERI_EARG, "liberi_c_get_stats: NULL argument");
   collect_timing(h);
   *out = h->stats;
+  for (int c = 0; c < kNumClasses; ++c) {
+    const ClassState &cs_ = h->cls[c];
+    const bool row = cs_.row_ok && (h->row_tiles == 2 || (h->row_tiles == 1 && cs_.variant == 1));
+    out->class_variant[c] = !cs_.active || cs_.variant < 0 ? -1 : row ? 1 + cs_.dev.transposed : 0;
+  }
   return LIBERI_OK;
 }
 
@@ -285,6 +324,77 @@ struct HostShell {
   const double *e, *w;
 };
 
+// Shells from the reference's setup arrays (src/liberi_shell_pair.F90:108-135),
+// with the argument checks the reference leaves to the caller.
+static int read_shells(const char *who, int32_t nsh, int32_t natoms, int32_t num_bas, int32_t mxgtot,
+                       const int32_t *ang_mom, const int32_t *contr_num, const int32_t *sh_loc,
+                       const int32_t *atom_num, const int32_t *atom_loc, const double *exponents, const double *cs,
+                       const double *cp, const double *cdc, const double *cf, const double *coords,
+                       std::vector<HostShell> &sh) {
+  const std::string w0 = std::string(who) + ": ";
+  if (nsh <= 0 || natoms <= 0 || num_bas <= 0 || mxgtot <= 0)
+    return fail(LIBERI_EARG, w0 + "nsh, natoms, num_bas, mxgtot must be positive");
+  if (!ang_mom || !contr_num || !sh_loc || !atom_num || !atom_loc || !exponents || !cs || !cp || !cdc || !cf ||
+      !coords)
+    return fail(LIBERI_EARG, w0 + "NULL array");
+  sh.assign(nsh, HostShell{});
+  const int64_t coef_len = (int64_t)mxgtot * nsh;
+  for (int s = 0; s < nsh; ++s) {
+    const int l = ang_mom[s] - 1, k = contr_num[s], loc = sh_loc[s] - 1, a = atom_num[s] - 1;
+    if (l < 0 || l > 3) return fail(LIBERI_EARG, w0 + "ang_mom must be 1..4 (shell " + std::to_string(s + 1) + ")");
+    if (k < 1 || loc < 0 || loc + k > coef_len)
+      return fail(LIBERI_EARG, w0 + "contr_num/sh_loc out of range (shell " + std::to_string(s + 1) + ")");
+    if (a < 0 || a >= natoms) return fail(LIBERI_EARG, w0 + "atom_num out of range");
+    const int bf0 = atom_loc[s] - 1;
+    if (bf0 < 0 || bf0 + (l + 1) * (l + 2) / 2 > num_bas) return fail(LIBERI_EARG, w0 + "atom_loc out of range");
+    const double *w = l == 0 ? cs : l == 1 ? cp : l == 2 ? cdc : cf;
+    for (int i = 0; i < k; ++i)
+      if (!(exponents[loc + i] > 0.0) || !std::isfinite(exponents[loc + i]) || !std::isfinite(w[loc + i]))
+        return fail(LIBERI_EARG, w0 + "exponents must be finite and > 0, coefficients finite (shell " +
+                                     std::to_string(s + 1) + ")");
+    sh[s] = HostShell{l, k, bf0, {coords[3 * a], coords[3 * a + 1], coords[3 * a + 2]}, exponents + loc, w + loc};
+  }
+  return LIBERI_OK;
+}
+
+static int ncart_host(int l) { return (l + 1) * (l + 2) / 2; }
+
+// Shell pair (sA, sB) and its primitive pairs appended to the device records
+// (the per-primitive-pair quantities of src/liberi_shell_pair.F90:416-459);
+// rank[s] = position of shell s among the shells of its angular momentum.
+static void append_pair(std::vector<PairRec> &pairs, std::vector<PrimRec> &prims, const std::vector<HostShell> &sh,
+                        const std::vector<int> *rank, int sA, int sB) {
+  const HostShell &A = sh[sA], &B = sh[sB];
+  PairRec r{};
+  r.prim_off = (int32_t)prims.size();
+  r.nprim = A.k * B.k;
+  r.bfA = A.bf0;
+  r.bfB = B.bf0;
+  r.ABx = A.c[0] - B.c[0];
+  r.ABy = A.c[1] - B.c[1];
+  r.ABz = A.c[2] - B.c[2];
+  r.sameAB = sA == sB;
+  r.shA = sA;
+  r.cA = rank ? (*rank)[sA] * ncart_host(A.l) : 0;
+  r.cB = rank ? (*rank)[sB] * ncart_host(B.l) : 0;
+  const double ab2 = r.ABx * r.ABx + r.ABy * r.ABy + r.ABz * r.ABz;
+  for (int i = 0; i < A.k; ++i)
+    for (int j = 0; j < B.k; ++j) {
+      const double a = A.e[i], b = B.e[j], p = a + b;
+      PrimRec pr;
+      pr.p = p;
+      pr.Px = (a * A.c[0] + b * B.c[0]) / p;
+      pr.Py = (a * A.c[1] + b * B.c[1]) / p;
+      pr.Pz = (a * A.c[2] + b * B.c[2]) / p;
+      pr.PAx = pr.Px - A.c[0];
+      pr.PAy = pr.Py - A.c[1];
+      pr.PAz = pr.Pz - A.c[2];
+      pr.K = A.w[i] * B.w[j] * std::exp(-a * b / p * ab2) / p;
+      prims.push_back(pr);
+    }
+  pairs.push_back(r);
+}
+
 int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, int32_t mxgtot,
                    const int32_t *ang_mom, const int32_t *contr_num, const int32_t *sh_loc, const int32_t *atom_num,
                    const int32_t *atom_loc, const int32_t *start_bas, const int32_t *end_bas,
@@ -293,33 +403,14 @@ int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, 
   (void)start_bas;  // stored but never read by the reference's hot path either
   (void)end_bas;
   if (!h) return fail(LIBERI_EARG, "liberi_c_setup: NULL handle");
-  if (nsh <= 0 || natoms <= 0 || num_bas <= 0 || mxgtot <= 0)
-    return fail(LIBERI_EARG, "liberi_c_setup: nsh, natoms, num_bas, mxgtot must be positive");
-  if (!ang_mom || !contr_num || !sh_loc || !atom_num || !atom_loc || !exponents || !cs || !cp || !cdc || !cf ||
-      !coords || !schwrz)
-    return fail(LIBERI_EARG, "liberi_c_setup: NULL array");
+  if (!schwrz) return fail(LIBERI_EARG, "liberi_c_setup: NULL array");
   if (num_procs < 1 || my_rank < 0 || my_rank >= num_procs)
     return fail(LIBERI_EARG, "liberi_c_setup: need 0 <= my_rank < num_procs");
+  std::vector<HostShell> sh;
+  if (int rc0 = read_shells("liberi_c_setup", nsh, natoms, num_bas, mxgtot, ang_mom, contr_num, sh_loc, atom_num,
+                            atom_loc, exponents, cs, cp, cdc, cf, coords, sh))
+    return rc0;
   if (h->initialized) liberi_c_cleanup(h);  // src/interface/liberi_interface.F90:133-136
-
-  // ---- shells (src/liberi_shell_pair.F90:108-135)
-  std::vector<HostShell> sh(nsh);
-  const int64_t coef_len = (int64_t)mxgtot * nsh;
-  for (int s = 0; s < nsh; ++s) {
-    const int l = ang_mom[s] - 1, k = contr_num[s], loc = sh_loc[s] - 1, a = atom_num[s] - 1;
-    if (l < 0 || l > 3) return fail(LIBERI_EARG, "liberi_c_setup: ang_mom must be 1..4 (shell " + std::to_string(s + 1) + ")");
-    if (k < 1 || loc < 0 || loc + k > coef_len)
-      return fail(LIBERI_EARG, "liberi_c_setup: contr_num/sh_loc out of range (shell " + std::to_string(s + 1) + ")");
-    if (a < 0 || a >= natoms) return fail(LIBERI_EARG, "liberi_c_setup: atom_num out of range");
-    const int bf0 = atom_loc[s] - 1;
-    if (bf0 < 0 || bf0 + (l + 1) * (l + 2) / 2 > num_bas) return fail(LIBERI_EARG, "liberi_c_setup: atom_loc out of range");
-    const double *w = l == 0 ? cs : l == 1 ? cp : l == 2 ? cdc : cf;
-    for (int i = 0; i < k; ++i)
-      if (!(exponents[loc + i] > 0.0) || !std::isfinite(exponents[loc + i]) || !std::isfinite(w[loc + i]))
-        return fail(LIBERI_EARG, "liberi_c_setup: exponents must be finite and > 0, coefficients finite (shell " +
-                                     std::to_string(s + 1) + ")");
-    sh[s] = HostShell{l, k, bf0, {coords[3 * a], coords[3 * a + 1], coords[3 * a + 2]}, exponents + loc, w + loc};
-  }
   h->nsh = nsh;
   h->natoms = natoms;
   h->num_bas = num_bas;
@@ -330,6 +421,19 @@ int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, 
   // ---- shell lists per L, ascending index (src/liberi_shell_pair.F90:137-344)
   std::vector<int> lists[4];
   for (int s = 0; s < nsh; ++s) lists[sh[s].l].push_back(s);
+  // compact columns: the functions of each angular momentum in shell order
+  std::vector<int> rank(nsh);
+  {
+    std::vector<int32_t> cols[4];
+    for (int l = 0; l < 4; ++l) {
+      for (size_t k = 0; k < lists[l].size(); ++k) {
+        rank[lists[l][k]] = (int)k;
+        for (int m = 0; m < ncart_host(l); ++m) cols[l].push_back(sh[lists[l][k]].bf0 + m);
+      }
+      h->ncol[l] = (int32_t)cols[l].size();
+      if (int rc1 = dev_upload(h, h->col_bf[l], cols[l])) return rc1;
+    }
+  }
 
   // ---- pair classes: records, Schwarz values (compute_*_pairs, :348-1682)
   std::vector<PairRec> pairs;
@@ -337,36 +441,11 @@ int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, 
   std::vector<double> pair_x;
   std::vector<std::vector<int32_t>> class_pairs(kNumPairClasses);
   auto add_pair = [&](int pc, int sA, int sB) {
-    const HostShell &A = sh[sA], &B = sh[sB];
-    PairRec r{};
-    r.prim_off = (int32_t)prims.size();
-    r.nprim = A.k * B.k;
-    r.bfA = A.bf0;
-    r.bfB = B.bf0;
-    r.ABx = A.c[0] - B.c[0];
-    r.ABy = A.c[1] - B.c[1];
-    r.ABz = A.c[2] - B.c[2];
-    r.sameAB = sA == sB;
-    r.shA = sA;
-    const double ab2 = r.ABx * r.ABx + r.ABy * r.ABy + r.ABz * r.ABz;
-    for (int i = 0; i < A.k; ++i)
-      for (int j = 0; j < B.k; ++j) {
-        const double a = A.e[i], b = B.e[j], p = a + b;
-        PrimRec pr;
-        pr.p = p;
-        pr.Px = (a * A.c[0] + b * B.c[0]) / p;
-        pr.Py = (a * A.c[1] + b * B.c[1]) / p;
-        pr.Pz = (a * A.c[2] + b * B.c[2]) / p;
-        pr.PAx = pr.Px - A.c[0];
-        pr.PAy = pr.Py - A.c[1];
-        pr.PAz = pr.Pz - A.c[2];
-        pr.K = A.w[i] * B.w[j] * std::exp(-a * b / p * ab2) / p;
-        prims.push_back(pr);
-      }
     const int hi = std::max(sA, sB), lo = std::min(sA, sB);
     pair_x.push_back(schwrz[(int64_t)hi * (hi + 1) / 2 + lo]);
     class_pairs[pc].push_back((int32_t)pairs.size());
-    pairs.push_back(r);
+    append_pair(pairs, prims, sh, &rank, sA, sB);
+    pairs.back().x = pair_x.back();
   };
   for (int pc = 0; pc < kNumPairClasses; ++pc) {
     const int l1 = kPairL[pc][0], l2 = kPairL[pc][1];
@@ -425,6 +504,7 @@ int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, 
   if ((rc = dev_upload(h, h->prims, prims))) return rc;
   const int64_t ntri = (int64_t)num_bas * (num_bas + 1) / 2;
   if ((rc = dev_alloc(h, h->dens, ntri * sizeof(double)))) return rc;
+  if ((rc = dev_alloc(h, h->aux, 2 * sizeof(double)))) return rc;
   if ((rc = dev_alloc(h, h->fock, ntri * sizeof(double)))) return rc;
 
   // ---- per class: pair pre-screen (host), quartet segments (device)
@@ -467,6 +547,21 @@ int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, 
     std::vector<int32_t> kb = keep(lb, sb);
     std::vector<int32_t> kk = diag ? kb : keep(lk, sk);
     if (kb.empty() || kk.empty()) continue;
+    // Orientation of the row tiles: the fixed pair of a tile keeps its
+    // exchange blocks on chip, the other pair's Coulomb block goes to F per
+    // quartet.  Fix the pair whose Coulomb block is larger, among the
+    // orientations whose accumulators fit in LDS.  The quartet set (and the
+    // count) is the same either way: the test is on the product x_bra*x_ket.
+    bool transposed = false;
+    {
+      const int nbra = ncart_host(kPairL[def.bra][0]) * ncart_host(kPairL[def.bra][1]);
+      const int nket = ncart_host(kPairL[def.ket][0]) * ncart_host(kPairL[def.ket][1]);
+      const int64_t l0 = class_row_lds(c, 0, h->ncol), l1 = diag ? -1 : class_row_lds(c, 1, h->ncol);
+      const bool fit0 = l0 >= 0 && l0 <= kRowLdsBytes, fit1 = l1 >= 0 && l1 <= kRowLdsBytes;
+      transposed = fit1 && (!fit0 || nket > nbra);
+      cs_.row_ok = fit0 || fit1;
+    }
+    if (transposed) kb.swap(kk);
     // ket groups (equal primitive count)
     std::vector<int32_t> gbeg, gend, kgroup(kk.size());
     for (size_t j = 0; j < kk.size(); ++j) {
@@ -510,10 +605,12 @@ int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, 
     cleanup_tmp();
     if (e != hipSuccess)
       return fail(LIBERI_EDEVICE, std::string("segment generation: ") + hipGetErrorString((hipError_t)e));
-    std::vector<int64_t> sstart;
-    std::vector<int32_t> sbra, sket0;
+    std::vector<int64_t> sstart, tq;
+    std::vector<int32_t> sbra, sket0, tseg;
     int64_t total = 0, primq = 0;
-    for (size_t i = 0; i < kb.size(); ++i)
+    for (size_t i = 0; i < kb.size(); ++i) {
+      const int64_t row0 = total;
+      const size_t sfirst = sstart.size();
       for (int g = 0; g < ng; ++g) {
         const int64_t L = len[i * ng + g];
         if (L <= 0) continue;
@@ -523,13 +620,25 @@ int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, 
         total += L;
         primq += L * (int64_t)pairs[kb[i]].nprim * pairs[kk[gbeg[g]]].nprim;
       }
+      // row tiles: the row's quartets in runs of at most kTileQuartets
+      size_t sg = sfirst;
+      for (int64_t t0 = row0; t0 < total; t0 += kTileQuartets) {
+        while (sg + 1 < sstart.size() && sstart[sg + 1] <= t0) ++sg;
+        tq.push_back(t0);
+        tseg.push_back((int32_t)sg);
+      }
+    }
     sstart.push_back(total);
+    tq.push_back(total);
     cs_.nquart = total;
     cs_.nprimq = primq;
     if (total == 0) continue;
+    if (sbra.size() >= (size_t)INT32_MAX || tq.size() >= (size_t)INT32_MAX)
+      return fail(LIBERI_ESIZE, std::string("class ") + def.name + ": too many segments");
     cs_.nseg = (int32_t)sbra.size();
     if ((rc = dev_upload(h, cs_.ket_list, kk)) || (rc = dev_upload(h, cs_.seg_start, sstart)) ||
-        (rc = dev_upload(h, cs_.seg_bra, sbra)) || (rc = dev_upload(h, cs_.seg_ket0, sket0)))
+        (rc = dev_upload(h, cs_.seg_bra, sbra)) || (rc = dev_upload(h, cs_.seg_ket0, sket0)) ||
+        (rc = dev_upload(h, cs_.tile_q, tq)) || (rc = dev_upload(h, cs_.tile_seg, tseg)))
       return rc;
     cs_.active = true;
     ClassDev &d = cs_.dev;
@@ -544,6 +653,16 @@ int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, 
     // class's index range the same way, e.g. src/rhf/rot_axis/int0001.F90:100-104)
     d.q_begin = total * my_rank / num_procs;
     d.q_end = total * (my_rank + 1) / num_procs;
+    // tiles overlapping [q_begin, q_end): tq is strictly increasing, tq.back() = total
+    const int32_t ntile = (int32_t)tq.size() - 1;
+    d.tile_q = (const int64_t *)cs_.tile_q.p;
+    d.tile_seg = (const int32_t *)cs_.tile_seg.p;
+    d.t_begin = (int32_t)(std::upper_bound(tq.begin(), tq.end(), d.q_begin) - tq.begin()) - 1;
+    d.t_end = (int32_t)(std::lower_bound(tq.begin(), tq.end(), d.q_end) - tq.begin());
+    d.t_begin = std::max<int32_t>(0, std::min(d.t_begin, ntile));
+    d.t_end = std::max(d.t_begin, std::min(d.t_end, ntile));
+    d.row_mode = 0;
+    d.transposed = transposed;
     d.xswitch = h->exact_large_t ? class_rys_xmax(c) : reference_switch(c);
     h->stats.quartets[c] = d.q_end - d.q_begin;
     h->stats.prim_quartets[c] = primq;
@@ -559,9 +678,58 @@ int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, 
 }
 
 // ------------------------------------------------------------------ build
+// Autotuner: for every class that has both a direct and a row-tiled kernel,
+// time each on this build's density (into a scratch Fock matrix, best of two
+// runs each) and keep the faster.  Runs once per setup, at the first build;
+// classes too small to time reliably, and every class when tuning is off, take
+// the default: row tiles for the lane-per-quartet classes (whose cost is the
+// atomics), direct digestion for the workgroup classes (whose occupancy the
+// LDS accumulators would cut).
+constexpr int64_t kTuneMinQuartets = 20000;
+
+static int choose_variants(liberi_ctx *h, const BuildArgs &ba, hipStream_t st) {
+  const int64_t ntri = (int64_t)h->num_bas * (h->num_bas + 1) / 2;
+  for (int c = 0; c < kNumClasses; ++c) {
+    ClassState &cs_ = h->cls[c];
+    if (!cs_.active || cs_.variant >= 0) continue;
+    const int dflt = (cs_.row_ok && class_kernel_kind(c) == 0) ? 1 : 0;
+    const int64_t nq = cs_.dev.q_end - cs_.dev.q_begin;
+    if (!h->autotune || h->row_tiles != 1 || !cs_.row_ok || nq < kTuneMinQuartets || cs_.dev.t_end <= cs_.dev.t_begin ||
+        (h->only_class >= 0 && c != h->only_class)) {
+      cs_.variant = dflt;
+      continue;
+    }
+    if (!h->tune_F.p)
+      if (int rc = dev_alloc(h, h->tune_F, ntri * sizeof(double))) return rc;
+    for (auto &e : h->tev)
+      if (!e) HIP_TRY(hipEventCreate(&e));
+    BuildArgs bt = ba;
+    bt.F = (double *)h->tune_F.p;
+    float best[2] = {1e30f, 1e30f};
+    for (int rep = 0; rep < 4; ++rep) {
+      const int v = rep & 1;
+      ClassDev cd = cs_.dev;
+      cd.row_mode = v;
+      HIP_TRY(hipEventRecord(h->tev[0], st));
+      int e = launch_class(c, cd, bt, st, kCoopWorkgroups);
+      if (e != hipSuccess)
+        return fail(LIBERI_EDEVICE, std::string("autotune ") + kClasses[c].name + ": " + hipGetErrorString((hipError_t)e));
+      HIP_TRY(hipEventRecord(h->tev[1], st));
+      HIP_TRY(hipEventSynchronize(h->tev[1]));
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, h->tev[0], h->tev[1]));
+      best[v] = std::min(best[v], ms);
+    }
+    cs_.variant = best[1] < best[0] ? 1 : 0;
+  }
+  return LIBERI_OK;
+}
+
 static int build_on_device(liberi_ctx *h, const double *dD, double *dF, hipStream_t st) {
   const int64_t ntri = (int64_t)h->num_bas * (h->num_bas + 1) / 2;
-  HIP_TRY(hipMemsetAsync(dF, 0, ntri * sizeof(double), st));  // fock = 0 (src/liberi_driver.F90:64)
+  // fock = 0 (src/liberi_driver.F90:64)
+  if (int e = launch_fill_zero(dF, ntri, st))
+    return fail(LIBERI_EDEVICE, std::string("zero F: ") + hipGetErrorString((hipError_t)e));
   BuildArgs ba;
   ba.pairs = (const PairRec *)h->pairs.p;
   ba.prims = (const PrimRec *)h->prims.p;
@@ -570,6 +738,20 @@ static int build_on_device(liberi_ctx *h, const double *dD, double *dF, hipStrea
   ba.value_cut = h->value_cut;
   ba.scratch = (double *)h->scratch.p;
   ba.scratch_per_wg = h->scratch_per_wg;
+  ba.nbf = h->num_bas;
+  ba.fp32_tau = h->fp32_tau;
+  ba.dmax = (const double *)h->aux.p;
+  ba.fp32_count = (uint64_t *)h->aux.p + 1;
+  if (h->fp32_tau > 0.0) {
+    int e = launch_fill_zero((double *)h->aux.p, 2, st);
+    if (e == hipSuccess) e = launch_abs_max(dD, ntri, (double *)h->aux.p, st);
+    if (e != hipSuccess) return fail(LIBERI_EDEVICE, std::string("max |D|: ") + hipGetErrorString((hipError_t)e));
+  }
+  for (int l = 0; l < 4; ++l) {
+    ba.col_bf[l] = (const int32_t *)h->col_bf[l].p;
+    ba.ncol[l] = h->ncol[l];
+  }
+  if (int rc = choose_variants(h, ba, st)) return rc;
   HIP_TRY(hipEventRecord(h->ev0, st));
   for (int c = 0; c < kNumClasses; ++c) {
     if (!h->cls[c].active) continue;
@@ -579,13 +761,18 @@ static int build_on_device(liberi_ctx *h, const double *dD, double *dF, hipStrea
       if (!h->cev1[c]) HIP_TRY(hipEventCreate(&h->cev1[c]));
       HIP_TRY(hipEventRecord(h->cev0[c], st));
     }
-    int e = launch_class(c, h->cls[c].dev, ba, st, kCoopWorkgroups);
+    ClassDev cd = h->cls[c].dev;
+    cd.row_mode = h->cls[c].row_ok && cd.t_end > cd.t_begin &&
+                  (h->row_tiles == 2 || (h->row_tiles == 1 && h->cls[c].variant == 1));
+    int e = launch_class(c, cd, ba, st, kCoopWorkgroups);
     if (e != hipSuccess)
       return fail(LIBERI_EDEVICE, std::string("launch ") + kClasses[c].name + ": " + hipGetErrorString((hipError_t)e));
     if (h->class_timing) HIP_TRY(hipEventRecord(h->cev1[c], st));
   }
   HIP_TRY(hipEventRecord(h->ev1, st));
   h->build_pending = true;
+  h->fp32_pending = h->fp32_tau > 0.0;
+  if (!h->fp32_pending) h->stats.fp32_quartets = 0;
   h->timing_pending = h->class_timing;
   if (!h->class_timing)
     for (double &v : h->stats.class_ms) v = 0.0;
@@ -653,6 +840,77 @@ int liberi_c_fock(const double *density, double *fock, int64_t n) {
 int liberi_c_finalize(void) {
   if (g_handle) liberi_c_destroy(&g_handle);
   g_matrix_size = 0;
+  return LIBERI_OK;
+}
+
+// ------------------------------------------------------ Schwarz integrals
+int liberi_c_schwarz(int32_t nsh, int32_t natoms, int32_t num_bas, int32_t mxgtot, const int32_t *ang_mom,
+                     const int32_t *contr_num, const int32_t *sh_loc, const int32_t *atom_num,
+                     const int32_t *atom_loc, const double *exponents, const double *cs, const double *cp,
+                     const double *cd, const double *cf, const double *coords, double *schwrz_int) {
+  if (!schwrz_int) return fail(LIBERI_EARG, "liberi_c_schwarz: NULL output");
+  std::vector<HostShell> sh;
+  if (int rc = read_shells("liberi_c_schwarz", nsh, natoms, num_bas, mxgtot, ang_mom, contr_num, sh_loc, atom_num,
+                           atom_loc, exponents, cs, cp, cd, cf, coords, sh))
+    return rc;
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+    return fail(LIBERI_EDEVICE, "liberi_c_schwarz: no HIP device visible");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(LIBERI_EDEVICE, std::string("liberi_c_schwarz: device is ") + prop.gcnArchName +
+                                    ", this library is built for gfx950 (MI355X)");
+  // every shell pair i >= j, A = the shell of higher angular momentum, grouped by (la, lb)
+  std::vector<PairRec> pairs;
+  std::vector<PrimRec> prims;
+  std::vector<int64_t> where;  // packed index of each pair, in pair order
+  std::vector<int32_t> by_l[16];
+  for (int i = 0; i < nsh; ++i)
+    for (int j = 0; j <= i; ++j) {
+      const int sA = sh[i].l >= sh[j].l ? i : j, sB = sA == i ? j : i;
+      by_l[sh[sA].l * 4 + sh[sB].l].push_back((int32_t)pairs.size());
+      where.push_back((int64_t)i * (i + 1) / 2 + j);
+      append_pair(pairs, prims, sh, nullptr, sA, sB);
+    }
+  if (prims.size() > (size_t)INT32_MAX || pairs.size() > (size_t)INT32_MAX)
+    return fail(LIBERI_EARG, "liberi_c_schwarz: basis too large for 32-bit pair indices");
+  std::vector<int32_t> list;
+  std::vector<size_t> off(17, 0);
+  for (int k = 0; k < 16; ++k) {
+    off[k] = list.size();
+    list.insert(list.end(), by_l[k].begin(), by_l[k].end());
+  }
+  off[16] = list.size();
+
+  void *dp = nullptr, *dq = nullptr, *dl = nullptr, *dout = nullptr;
+  hipStream_t st = nullptr;
+  auto release = [&]() {
+    if (st) (void)hipStreamDestroy(st);
+    for (void *p : {dp, dq, dl, dout})
+      if (p) (void)hipFree(p);
+  };
+  std::vector<double> out(list.size());
+  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&dp, pairs.size() * sizeof(PairRec));
+  if (e == hipSuccess) e = hipMalloc(&dq, prims.size() * sizeof(PrimRec));
+  if (e == hipSuccess) e = hipMalloc(&dl, list.size() * sizeof(int32_t));
+  if (e == hipSuccess) e = hipMalloc(&dout, list.size() * sizeof(double));
+  if (e == hipSuccess) e = hipMemcpyAsync(dp, pairs.data(), pairs.size() * sizeof(PairRec), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dq, prims.data(), prims.size() * sizeof(PrimRec), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dl, list.data(), list.size() * sizeof(int32_t), hipMemcpyHostToDevice, st);
+  for (int k = 0; k < 16 && e == hipSuccess; ++k) {
+    const int32_t n = (int32_t)(off[k + 1] - off[k]);
+    if (n > 0)
+      e = (hipError_t)launch_schwarz(k / 4, k % 4, (const PairRec *)dp, (const PrimRec *)dq,
+                                     (const int32_t *)dl + off[k], n, (double *)dout + off[k], st);
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out.data(), dout, out.size() * sizeof(double), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  release();
+  if (e != hipSuccess)
+    return fail(e == hipErrorOutOfMemory ? LIBERI_ENOMEM : LIBERI_EDEVICE,
+                std::string("liberi_c_schwarz: ") + hipGetErrorString(e));
+  for (size_t k = 0; k < list.size(); ++k) schwrz_int[where[list[k]]] = out[k];
   return LIBERI_OK;
 }
 

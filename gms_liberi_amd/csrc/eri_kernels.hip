@@ -155,26 +155,25 @@ __device__ __forceinline__ void rys_root(double X, double xswitch, int r, double
 // 2-D Rys integrals G(i,k), i <= NI-1 (bra), k <= NK-1 (ket), for one root and
 // one cartesian direction:  G(i+1,0) = C00 G(i,0) + i B10 G(i-1,0),
 // G(i,k+1) = C00' G(i,k) + k B01 G(i,k-1) + i B00 G(i-1,k).
-template <int NI, int NK>
-__device__ __forceinline__ void rys_2d(double (&g)[NI][NK], double c00, double c00p, double b10, double b01,
-                                       double b00, double g00) {
+template <int NI, int NK, class T>
+__device__ __forceinline__ void rys_2d(T (&g)[NI][NK], T c00, T c00p, T b10, T b01, T b00, T g00) {
   g[0][0] = g00;
   if (NI > 1) g[1][0] = c00 * g00;
 #pragma unroll
-  for (int i = 1; i + 1 < NI; ++i) g[i + 1][0] = c00 * g[i][0] + (double)i * b10 * g[i - 1][0];
+  for (int i = 1; i + 1 < NI; ++i) g[i + 1][0] = c00 * g[i][0] + (T)i * b10 * g[i - 1][0];
   if (NK > 1) {
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
-      double v = c00p * g[i][0];
-      if (i > 0) v += (double)i * b00 * g[i - 1][0];
+      T v = c00p * g[i][0];
+      if (i > 0) v += (T)i * b00 * g[i - 1][0];
       g[i][1] = v;
     }
 #pragma unroll
     for (int k = 1; k + 1 < NK; ++k)
 #pragma unroll
       for (int i = 0; i < NI; ++i) {
-        double v = c00p * g[i][k] + (double)k * b01 * g[i][k - 1];
-        if (i > 0) v += (double)i * b00 * g[i - 1][k];
+        T v = c00p * g[i][k] + (T)k * b01 * g[i][k - 1];
+        if (i > 0) v += (T)i * b00 * g[i - 1][k];
         g[i][k + 1] = v;
       }
   }
@@ -194,8 +193,10 @@ __device__ __forceinline__ int find_segment(const ClassDev &cd, int64_t q, int l
 
 __device__ __forceinline__ void quartet_of_segment(const ClassDev &cd, int s, int64_t q, int32_t &bra,
                                                    int32_t &ket) {
-  bra = cd.seg_bra[s];
-  ket = cd.ket_list[cd.seg_ket0[s] + (int32_t)(q - cd.seg_start[s])];
+  const int32_t row = cd.seg_bra[s];
+  const int32_t col = cd.ket_list[cd.seg_ket0[s] + (int32_t)(q - cd.seg_start[s])];
+  bra = cd.transposed ? col : row;
+  ket = cd.transposed ? row : col;
 }
 
 // quartet index -> (bra pair, ket pair) through the class's segment table
@@ -208,8 +209,7 @@ __device__ __forceinline__ void decode_quartet(const ClassDev &cd, int64_t q, in
     else
       hi = mid;
   }
-  bra = cd.seg_bra[lo];
-  ket = cd.ket_list[cd.seg_ket0[lo] + (int32_t)(q - cd.seg_start[lo])];
+  quartet_of_segment(cd, lo, q, bra, ket);
 }
 
 __device__ __forceinline__ void fock_add(double *F, int64_t i, int64_t j, double v) {
@@ -238,12 +238,13 @@ struct Small {
   static constexpr int NR = (LAB + LCD) / 2 + 1;
   static constexpr int NE = nrange(LA, LAB), NF = nrange(LC, LCD);
   static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  static constexpr int NABCD = NA * NB * NC * ND;
   static constexpr int NBUF = hrr_max_buffer(LA, LB, LC, LD);
 };
 
 // bra HRR level J: out[a in range(LA, LAB-J)][b in cart(J)][f] from level J-1
-template <int LA, int LAB, int J, int NF>
-__device__ __forceinline__ void hrr_bra_level(const double *in, double *out, double abx, double aby, double abz) {
+template <int LA, int LAB, int J, int NF, class T>
+__device__ __forceinline__ void hrr_bra_level(const T *in, T *out, T abx, T aby, T abz) {
   constexpr Range<LA, LAB - J> ra{};
   constexpr Range<J, J> rb{};
   constexpr Range<J - 1, J - 1> rbm{};
@@ -258,15 +259,15 @@ __device__ __forceinline__ void hrr_bra_level(const double *in, double *out, dou
       const int bmi = cidx(bx - (dir == 0), by - (dir == 1), bz - (dir == 2));
       const int ax = ra.lx[a] + (dir == 0), ay = ra.ly[a] + (dir == 1), az = ra.lz[a] + (dir == 2);
       const int api = nrange(LA, ra.l[a]) + cidx(ax, ay, az);  // index in range(LA, LAB-J+1)
-      const double ab = dir == 0 ? abx : (dir == 1 ? aby : abz);
+      const T ab = dir == 0 ? abx : (dir == 1 ? aby : abz);
 #pragma unroll
       for (int f = 0; f < NF; ++f)
         out[(a * nb + b) * NF + f] = in[(api * nbm + bmi) * NF + f] + ab * in[(a * nbm + bmi) * NF + f];
     }
 }
 
-template <int LA, int LB, int NF>
-__device__ __forceinline__ void hrr_bra(double *buf0, double *buf1, double abx, double aby, double abz) {
+template <int LA, int LB, int NF, class T>
+__device__ __forceinline__ void hrr_bra(T *buf0, T *buf1, T abx, T aby, T abz) {
   // level 0 lives in buf0 as [e in range(LA, LA+LB)][1][f]; result in buf0 when LB is even, buf1 when odd
   if constexpr (LB >= 1) hrr_bra_level<LA, LA + LB, 1, NF>(buf0, buf1, abx, aby, abz);
   if constexpr (LB >= 2) hrr_bra_level<LA, LA + LB, 2, NF>(buf1, buf0, abx, aby, abz);
@@ -274,8 +275,8 @@ __device__ __forceinline__ void hrr_bra(double *buf0, double *buf1, double abx, 
 }
 
 // ket HRR level J on the inner index: out[ab][c in range(LC, LCD-J)][d in cart(J)]
-template <int LC, int LCD, int J, int NAB>
-__device__ __forceinline__ void hrr_ket_level(const double *in, double *out, double cdx, double cdy, double cdz) {
+template <int LC, int LCD, int J, int NAB, class T>
+__device__ __forceinline__ void hrr_ket_level(const T *in, T *out, T cdx, T cdy, T cdz) {
   constexpr Range<LC, LCD - J> rc{};
   constexpr Range<J, J> rd{};
   constexpr int nin = nrange(LC, LCD - J + 1) * ncart(J - 1);
@@ -290,28 +291,130 @@ __device__ __forceinline__ void hrr_ket_level(const double *in, double *out, dou
       const int dmi = cidx(dx - (dir == 0), dy - (dir == 1), dz - (dir == 2));
       const int cx = rc.lx[c] + (dir == 0), cy = rc.ly[c] + (dir == 1), cz = rc.lz[c] + (dir == 2);
       const int cpi = nrange(LC, rc.l[c]) + cidx(cx, cy, cz);
-      const double cdv = dir == 0 ? cdx : (dir == 1 ? cdy : cdz);
+      const T cdv = dir == 0 ? cdx : (dir == 1 ? cdy : cdz);
 #pragma unroll
       for (int ab = 0; ab < NAB; ++ab)
         out[ab * nout + c * nd + d] = in[ab * nin + cpi * ndm + dmi] + cdv * in[ab * nin + c * ndm + dmi];
     }
 }
 
-template <int LC, int LD, int NAB>
-__device__ __forceinline__ void hrr_ket(double *buf0, double *buf1, double cdx, double cdy, double cdz) {
+template <int LC, int LD, int NAB, class T>
+__device__ __forceinline__ void hrr_ket(T *buf0, T *buf1, T cdx, T cdy, T cdz) {
   if constexpr (LD >= 1) hrr_ket_level<LC, LC + LD, 1, NAB>(buf0, buf1, cdx, cdy, cdz);
   if constexpr (LD >= 2) hrr_ket_level<LC, LC + LD, 2, NAB>(buf1, buf0, cdx, cdy, cdz);
   if constexpr (LD >= 3) hrr_ket_level<LC, LC + LD, 3, NAB>(buf0, buf1, cdx, cdy, cdz);
 }
 
-template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__(256) void eri_small_kernel(ClassDev cd, BuildArgs ba) {
+// Mixed precision: a quartet whose Schwarz bound times max|D| is below tau is
+// evaluated in FP32 (tau = 0: never).  The count of such quartets is kept per
+// wave (one atomic add per wave).
+__device__ __forceinline__ bool mixed_low(const BuildArgs &ba, const PairRec &bra, const PairRec &ket) {
+  if (!(ba.fp32_tau > 0.0)) return false;
+  const bool lo = bra.x * ket.x * ba.dmax[0] < ba.fp32_tau;
+  const uint64_t m = __ballot(lo);
+  if (m != 0 && __lane_id() == __ffsll((unsigned long long)m) - 1)
+    atomicAdd((unsigned long long *)ba.fp32_count, (unsigned long long)__popcll(m));
+  return lo;
+}
+
+// One contracted quartet held by one lane: primitive loop with Rys quadrature,
+// contracted HRR, cartesian scale factors and the value cutoff.  Result
+// v[((a*NB + b)*NC + c)*ND + d], components in internal order.  T is the
+// arithmetic type of the 2-D integrals, the primitive accumulation and the
+// HRR; the Rys argument, roots and weights are always FP64 (they stand where
+// the reference evaluates its Boys function), and so is the result.
+template <class T, int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void small_integrals(const PairRec &bra, const PairRec &ket, const PrimRec *prims,
+                                                double xswitch, double cut,
+                                                double (&v)[Small<LA, LB, LC, LD>::NABCD]) {
   using S = Small<LA, LB, LC, LD>;
   constexpr int NE = S::NE, NF = S::NF, NR = S::NR, NI = S::LAB + 1, NK = S::LCD + 1;
   constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND;
   constexpr int NBUF = S::NBUF;
   constexpr Range<LA, LA + LB> re{};
   constexpr Range<LC, LC + LD> rf{};
+
+  T acc[NBUF];
+#pragma unroll
+  for (int i = 0; i < NE * NF; ++i) acc[i] = (T)0;
+
+  for (int ip = 0; ip < bra.nprim; ++ip) {
+    const PrimRec P = prims[bra.prim_off + ip];
+    for (int iq = 0; iq < ket.nprim; ++iq) {
+      const PrimRec Q = prims[ket.prim_off + iq];
+      const double pq = P.p + Q.p, ipq = 1.0 / pq;
+      const double pqx = P.Px - Q.Px, pqy = P.Py - Q.Py, pqz = P.Pz - Q.Pz;
+      const double rho = P.p * Q.p * ipq;
+      const double X = rho * (pqx * pqx + pqy * pqy + pqz * pqz);
+      const double pref = kTwoPi52 * P.K * Q.K * sqrt(ipq);
+      double u[NR], w[NR];
+      rys_rule<NR>(X, xswitch, u, w);
+      const T tp = (T)P.p, tq = (T)Q.p, tipq = (T)ipq;
+      const T hp = (T)0.5 / tp, hq = (T)0.5 / tq;
+      const T tpqx = (T)pqx, tpqy = (T)pqy, tpqz = (T)pqz;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const T t2 = (T)u[r];
+        const T b00 = (T)0.5 * t2 * tipq;
+        const T b10 = hp - hp * tq * t2 * tipq;
+        const T b01 = hq - hq * tp * t2 * tipq;
+        const T fa = tq * tipq * t2, fc = tp * tipq * t2;
+        T gx[NI][NK], gy[NI][NK], gz[NI][NK];
+        rys_2d<NI, NK>(gx, (T)P.PAx - fa * tpqx, (T)Q.PAx + fc * tpqx, b10, b01, b00, (T)1);
+        rys_2d<NI, NK>(gy, (T)P.PAy - fa * tpqy, (T)Q.PAy + fc * tpqy, b10, b01, b00, (T)1);
+        rys_2d<NI, NK>(gz, (T)P.PAz - fa * tpqz, (T)Q.PAz + fc * tpqz, b10, b01, b00, (T)(w[r] * pref));
+#pragma unroll
+        for (int e = 0; e < NE; ++e)
+#pragma unroll
+          for (int f = 0; f < NF; ++f)
+            acc[e * NF + f] += gx[re.lx[e]][rf.lx[f]] * gy[re.ly[e]][rf.ly[f]] * gz[re.lz[e]][rf.lz[f]];
+      }
+    }
+  }
+
+  // contracted HRR: bra then ket, ping-pong between two register arrays; the
+  // result ends in b1 when exactly one of LB, LD is odd
+  T b1[NBUF];
+  if constexpr (LB > 0) hrr_bra<LA, LB, NF>(acc, b1, (T)bra.ABx, (T)bra.ABy, (T)bra.ABz);
+  if constexpr (LD > 0) {
+    if constexpr (LB % 2)
+      hrr_ket<LC, LD, NA * NB>(b1, acc, (T)ket.ABx, (T)ket.ABy, (T)ket.ABz);
+    else
+      hrr_ket<LC, LD, NA * NB>(acc, b1, (T)ket.ABx, (T)ket.ABy, (T)ket.ABz);
+  }
+  constexpr bool in_b1 = ((LB % 2) != 0) != ((LD % 2) != 0);
+
+  // normalised block v[a][b][c][d] (internal order), value cutoff
+  constexpr Range<LA, LA> ra{};
+  constexpr Range<LB, LB> rb{};
+  constexpr Range<LC, LC> rc{};
+  constexpr Range<LD, LD> rd{};
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+          const double sc = cart_scale(ra.lx[a], ra.ly[a], ra.lz[a]) * cart_scale(rb.lx[b], rb.ly[b], rb.lz[b]) *
+                            cart_scale(rc.lx[c], rc.ly[c], rc.lz[c]) * cart_scale(rd.lx[d], rd.ly[d], rd.lz[d]);
+          const int idx = ((a * NB + b) * NC + c) * ND + d;
+          const double x = (double)(in_b1 ? b1[idx] : acc[idx]) * sc;
+          v[idx] = fabs(x) < cut ? 0.0 : x;
+        }
+}
+
+// Lane-per-quartet kernel with direct digestion: every Fock element of the
+// six blocks of a quartet is one atomic add to F.
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void eri_small_kernel(ClassDev cd, BuildArgs ba) {
+  using S = Small<LA, LB, LC, LD>;
+  constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND;
+  constexpr Range<LA, LA> ra{};
+  constexpr Range<LB, LB> rb{};
+  constexpr Range<LC, LC> rc{};
+  constexpr Range<LD, LD> rd{};
 
   // the block's quartets span segments [s_lo, s_hi]: two full searches, then
   // every lane searches only that short range
@@ -327,76 +430,11 @@ __global__ __launch_bounds__(256) void eri_small_kernel(ClassDev cd, BuildArgs b
   quartet_of_segment(cd, find_segment(cd, q, s_lo, s_hi + 1), q, ib, ik);
   const PairRec bra = ba.pairs[ib];
   const PairRec ket = ba.pairs[ik];
-
-  double acc[NBUF];
-#pragma unroll
-  for (int i = 0; i < NE * NF; ++i) acc[i] = 0.0;
-
-  for (int ip = 0; ip < bra.nprim; ++ip) {
-    const PrimRec P = ba.prims[bra.prim_off + ip];
-    for (int iq = 0; iq < ket.nprim; ++iq) {
-      const PrimRec Q = ba.prims[ket.prim_off + iq];
-      const double pq = P.p + Q.p, ipq = 1.0 / pq;
-      const double pqx = P.Px - Q.Px, pqy = P.Py - Q.Py, pqz = P.Pz - Q.Pz;
-      const double rho = P.p * Q.p * ipq;
-      const double X = rho * (pqx * pqx + pqy * pqy + pqz * pqz);
-      const double pref = kTwoPi52 * P.K * Q.K * sqrt(ipq);
-      double u[NR], w[NR];
-      rys_rule<NR>(X, cd.xswitch, u, w);
-      const double hp = 0.5 / P.p, hq = 0.5 / Q.p;
-#pragma unroll
-      for (int r = 0; r < NR; ++r) {
-        const double t2 = u[r];
-        const double b00 = 0.5 * t2 * ipq;
-        const double b10 = hp - hp * Q.p * t2 * ipq;
-        const double b01 = hq - hq * P.p * t2 * ipq;
-        const double fa = Q.p * ipq * t2, fc = P.p * ipq * t2;
-        double gx[NI][NK], gy[NI][NK], gz[NI][NK];
-        rys_2d<NI, NK>(gx, P.PAx - fa * pqx, Q.PAx + fc * pqx, b10, b01, b00, 1.0);
-        rys_2d<NI, NK>(gy, P.PAy - fa * pqy, Q.PAy + fc * pqy, b10, b01, b00, 1.0);
-        rys_2d<NI, NK>(gz, P.PAz - fa * pqz, Q.PAz + fc * pqz, b10, b01, b00, w[r] * pref);
-#pragma unroll
-        for (int e = 0; e < NE; ++e)
-#pragma unroll
-          for (int f = 0; f < NF; ++f)
-            acc[e * NF + f] += gx[re.lx[e]][rf.lx[f]] * gy[re.ly[e]][rf.ly[f]] * gz[re.lz[e]][rf.lz[f]];
-      }
-    }
-  }
-
-  // contracted HRR: bra then ket (in-place ping-pong through two register arrays)
-  double b1[NBUF];
-  double *res = acc;
-  if constexpr (LB > 0) {
-    hrr_bra<LA, LB, NF>(acc, b1, bra.ABx, bra.ABy, bra.ABz);
-    res = (LB % 2) ? b1 : acc;
-  }
-  if constexpr (LD > 0) {
-    double *other = (res == acc) ? b1 : acc;
-    hrr_ket<LC, LD, NA * NB>(res, other, ket.ABx, ket.ABy, ket.ABz);
-    if (LD % 2) res = other;
-  }
-
-  // normalised block v[a][b][c][d] (internal order), value cutoff
-  constexpr Range<LA, LA> ra{};
-  constexpr Range<LB, LB> rb{};
-  constexpr Range<LC, LC> rc{};
-  constexpr Range<LD, LD> rd{};
-  const double cut = ba.value_cut;
-#pragma unroll
-  for (int a = 0; a < NA; ++a)
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
-#pragma unroll
-      for (int c = 0; c < NC; ++c)
-#pragma unroll
-        for (int d = 0; d < ND; ++d) {
-          const double sc = cart_scale(ra.lx[a], ra.ly[a], ra.lz[a]) * cart_scale(rb.lx[b], rb.ly[b], rb.lz[b]) *
-                            cart_scale(rc.lx[c], rc.ly[c], rc.lz[c]) * cart_scale(rd.lx[d], rd.ly[d], rd.lz[d]);
-          const int idx = ((a * NB + b) * NC + c) * ND + d;
-          const double v = res[idx] * sc;
-          res[idx] = fabs(v) < cut ? 0.0 : v;
-        }
+  double res[S::NABCD];
+  if (mixed_low(ba, bra, ket))
+    small_integrals<float, LA, LB, LC, LD>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, res);
+  else
+    small_integrals<double, LA, LB, LC, LD>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, res);
 
   // degeneracy weight of the shell quartet (A==B, C==D, AB==CD)
   double s = 1.0;
@@ -514,6 +552,252 @@ __global__ __launch_bounds__(256) void eri_small_kernel(ClassDev cd, BuildArgs b
           for (int d = 0; d < ND; ++d) t += res[((a * NB + b) * NC + c) * ND + d] * dad[a * ND + d];
         if (t != 0.0) fock_add(F, gb[b], gc[c], -s * t);
       }
+  }
+}
+
+__device__ __forceinline__ int gpos_of(int L, int k);
+
+__device__ __forceinline__ void lds_add(double *p, double v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Shape of a row tile of class (LA LB | LC LD).  The tile fixes one shell
+// pair (the bra pair, or with TR the ket pair): its shells give the rows of
+// the on-chip Fock accumulators.  The other ("moving") pair varies per
+// quartet; its shells' angular momenta M1, M2 give the column spaces, each the
+// compact list of all functions of that angular momentum (one space when
+// M1 == M2).
+template <int LA, int LB, int LC, int LD, bool TR>
+struct RowShape {
+  static constexpr int L1 = TR ? LC : LA, L2 = TR ? LD : LB;
+  static constexpr int M1 = TR ? LA : LC, M2 = TR ? LB : LD;
+  static constexpr int N1 = ncart(L1), N2 = ncart(L2), NROW = N1 + N2, NFIX = N1 * N2;
+  static constexpr int NM1 = ncart(M1), NM2 = ncart(M2);
+  // a class whose bra and ket have the same shape is diagonal: no TR form
+  static constexpr bool exists = !(TR && LA == LC && LB == LD);
+};
+
+template <int LA, int LB, int LC, int LD, bool TR>
+__host__ __device__ inline int64_t row_columns(const int32_t *ncol) {
+  using R = RowShape<LA, LB, LC, LD, TR>;
+  return (int64_t)ncol[R::M1] + (R::M1 == R::M2 ? 0 : ncol[R::M2]);
+}
+
+// Row-tiled lane-per-quartet kernel.  One workgroup owns one tile: a run of
+// quartets that share the fixed pair.  Four of the six digestion blocks --
+// the exchange blocks K(AC), K(AD), K(BC), K(BD) -- land in Fock rows of the
+// fixed pair's shells; they are accumulated in LDS, rows[r][column], and
+// added to F once per tile.  The Coulomb block of the fixed pair is summed in
+// registers over the tile and reduced across the workgroup at the end.  Only
+// the Coulomb block of the moving pair goes to F per quartet.
+template <int LA, int LB, int LC, int LD, bool TR>
+__global__ __launch_bounds__(256) void eri_row_kernel(ClassDev cd, BuildArgs ba) {
+  using S = Small<LA, LB, LC, LD>;
+  using R = RowShape<LA, LB, LC, LD, TR>;
+  constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND;
+  constexpr int N1 = R::N1, N2 = R::N2, NFIX = R::NFIX, NROW = R::NROW;
+  constexpr Range<LA, LA> ra{};
+  constexpr Range<LB, LB> rb{};
+  constexpr Range<LC, LC> rc{};
+  constexpr Range<LD, LD> rd{};
+  constexpr Range<R::L1, R::L1> r1{};
+  constexpr Range<R::L2, R::L2> r2{};
+  constexpr Range<R::M1, R::M1> rm1{};
+  constexpr Range<R::M2, R::M2> rm2{};
+  extern __shared__ double lds[];  // jfix[NFIX], then rows[NROW][ncols]
+  double *jfix_s = lds, *rows = lds + NFIX;
+  const int ncols = (int)row_columns<LA, LB, LC, LD, TR>(ba.ncol);
+  const int off2 = R::M1 == R::M2 ? 0 : ba.ncol[R::M1];
+
+  const int tile = cd.t_begin + (int)blockIdx.x;
+  const int64_t q0 = cd.tile_q[tile] > cd.q_begin ? cd.tile_q[tile] : cd.q_begin;
+  const int64_t q1 = cd.tile_q[tile + 1] < cd.q_end ? cd.tile_q[tile + 1] : cd.q_end;
+  const int seg0 = cd.tile_seg[tile];
+  const int32_t ifix = cd.seg_bra[seg0];
+  const PairRec fix = ba.pairs[ifix];
+  for (int i = threadIdx.x; i < NFIX + NROW * ncols; i += blockDim.x) lds[i] = 0.0;
+
+  const double *Dm = ba.D;
+  double *F = ba.F;
+  // the fixed pair's density block and Coulomb partial sums
+  double dfix[NFIX], jfix[NFIX];
+#pragma unroll
+  for (int i = 0; i < N1; ++i)
+#pragma unroll
+    for (int j = 0; j < N2; ++j) {
+      dfix[i * N2 + j] = Dm[tri(fix.bfA + gamess_pos(r1.lx[i], r1.ly[i], r1.lz[i]),
+                                fix.bfB + gamess_pos(r2.lx[j], r2.ly[j], r2.lz[j]))];
+      jfix[i * N2 + j] = 0.0;
+    }
+  const double sfix = fix.sameAB ? 0.5 : 1.0;
+  __syncthreads();
+
+  for (int64_t qb = q0; qb < q1; qb += blockDim.x) {
+    const int64_t q = qb + threadIdx.x;
+    if (q >= q1) continue;
+    int sg = seg0;
+    while (sg + 1 < cd.nseg && cd.seg_start[sg + 1] <= q) ++sg;
+    const int32_t imov = cd.ket_list[cd.seg_ket0[sg] + (int32_t)(q - cd.seg_start[sg])];
+    const PairRec mov = ba.pairs[imov];
+    const PairRec &bra = TR ? mov : fix;
+    const PairRec &ket = TR ? fix : mov;
+    double v[S::NABCD];
+    if (mixed_low(ba, bra, ket))
+      small_integrals<float, LA, LB, LC, LD>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, v);
+    else
+      small_integrals<double, LA, LB, LC, LD>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, v);
+
+    double s = sfix;
+    if (mov.sameAB) s *= 0.5;
+    if (cd.diag && ifix == imov) s *= 0.5;
+    int ga[NA], gb[NB], gc[NC], gd[ND];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) ga[a] = bra.bfA + gamess_pos(ra.lx[a], ra.ly[a], ra.lz[a]);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) gb[b] = bra.bfB + gamess_pos(rb.lx[b], rb.ly[b], rb.lz[b]);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) gc[c] = ket.bfA + gamess_pos(rc.lx[c], rc.ly[c], rc.lz[c]);
+#pragma unroll
+    for (int d = 0; d < ND; ++d) gd[d] = ket.bfB + gamess_pos(rd.lx[d], rd.ly[d], rd.lz[d]);
+    // compact columns of the moving pair's functions
+    int cm1[R::NM1], cm2[R::NM2];
+#pragma unroll
+    for (int i = 0; i < R::NM1; ++i) cm1[i] = mov.cA + gamess_pos(rm1.lx[i], rm1.ly[i], rm1.lz[i]);
+#pragma unroll
+    for (int i = 0; i < R::NM2; ++i) cm2[i] = off2 + mov.cB + gamess_pos(rm2.lx[i], rm2.ly[i], rm2.lz[i]);
+
+    // Coulomb: the fixed pair's block in registers, the moving pair's to F
+    if constexpr (!TR) {
+      double dcd[NC * ND];
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) dcd[c * ND + d] = Dm[tri(gc[c], gd[d])];
+#pragma unroll
+      for (int ab = 0; ab < NA * NB; ++ab) {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < NC * ND; ++k) t += v[ab * NC * ND + k] * dcd[k];
+        jfix[ab] += 4.0 * s * t;
+      }
+#pragma unroll
+      for (int k = 0; k < NC * ND; ++k) {
+        double t = 0.0;
+#pragma unroll
+        for (int ab = 0; ab < NA * NB; ++ab) t += v[ab * NC * ND + k] * dfix[ab];
+        if (t != 0.0) fock_add(F, gc[k / ND], gd[k % ND], 4.0 * s * t);
+      }
+    } else {
+      double dab[NA * NB];
+#pragma unroll
+      for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) dab[a * NB + b] = Dm[tri(ga[a], gb[b])];
+#pragma unroll
+      for (int k = 0; k < NC * ND; ++k) {
+        double t = 0.0;
+#pragma unroll
+        for (int ab = 0; ab < NA * NB; ++ab) t += v[ab * NC * ND + k] * dab[ab];
+        jfix[k] += 4.0 * s * t;
+      }
+#pragma unroll
+      for (int ab = 0; ab < NA * NB; ++ab) {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < NC * ND; ++k) t += v[ab * NC * ND + k] * dfix[k];
+        if (t != 0.0) fock_add(F, ga[ab / NB], gb[ab % NB], 4.0 * s * t);
+      }
+    }
+    // exchange: K(AC) and K(BD)
+    {
+      double dbd[NB * ND], dac[NA * NC];
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) dbd[b * ND + d] = Dm[tri(gb[b], gd[d])];
+#pragma unroll
+      for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) dac[a * NC + c] = Dm[tri(ga[a], gc[c])];
+#pragma unroll
+      for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          double t = 0.0;
+#pragma unroll
+          for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) t += v[((a * NB + b) * NC + c) * ND + d] * dbd[b * ND + d];
+          if (t != 0.0) lds_add(TR ? &rows[c * ncols + cm1[a]] : &rows[a * ncols + cm1[c]], -s * t);
+        }
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+          double t = 0.0;
+#pragma unroll
+          for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) t += v[((a * NB + b) * NC + c) * ND + d] * dac[a * NC + c];
+          if (t != 0.0) lds_add(TR ? &rows[(NC + d) * ncols + cm2[b]] : &rows[(NA + b) * ncols + cm2[d]], -s * t);
+        }
+    }
+    // exchange: K(AD) and K(BC)
+    {
+      double dbc[NB * NC], dad[NA * ND];
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) dbc[b * NC + c] = Dm[tri(gb[b], gc[c])];
+#pragma unroll
+      for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) dad[a * ND + d] = Dm[tri(ga[a], gd[d])];
+#pragma unroll
+      for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+          double t = 0.0;
+#pragma unroll
+          for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) t += v[((a * NB + b) * NC + c) * ND + d] * dbc[b * NC + c];
+          if (t != 0.0) lds_add(TR ? &rows[(NC + d) * ncols + cm1[a]] : &rows[a * ncols + cm2[d]], -s * t);
+        }
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          double t = 0.0;
+#pragma unroll
+          for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) t += v[((a * NB + b) * NC + c) * ND + d] * dad[a * ND + d];
+          if (t != 0.0) lds_add(TR ? &rows[c * ncols + cm2[b]] : &rows[(NA + b) * ncols + cm1[c]], -s * t);
+        }
+    }
+  }
+
+  // the fixed pair's Coulomb block: reduce over each wave, then over the waves
+#pragma unroll
+  for (int k = 0; k < NFIX; ++k) {
+    double x = jfix[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    if ((threadIdx.x & 63) == 0 && x != 0.0) lds_add(&jfix_s[k], x);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < NFIX; i += blockDim.x) {
+    const double x = jfix_s[i];
+    if (x != 0.0) fock_add(F, fix.bfA + gpos_of(R::L1, i / N2), fix.bfB + gpos_of(R::L2, i % N2), x);
+  }
+  for (int i = threadIdx.x; i < NROW * ncols; i += blockDim.x) {
+    const double x = rows[i];
+    if (x == 0.0) continue;
+    const int r = i / ncols, k = i - r * ncols;
+    const int rbf = r < N1 ? fix.bfA + gpos_of(R::L1, r) : fix.bfB + gpos_of(R::L2, r - N1);
+    const int cbf = (R::M1 == R::M2 || k < off2) ? ba.col_bf[R::M1][k] : ba.col_bf[R::M2][k - off2];
+    fock_add(F, rbf, cbf, x);
   }
 }
 
@@ -853,14 +1137,21 @@ __device__ __forceinline__ int gpos_of(int L, int k) {
   return gamess_pos(x, y, z);
 }
 
-template <int LA, int LB, int LC, int LD>
+// MODE 0: persistent loop over the class's quartets, digestion by atomic adds
+// to F.  MODE 1 / 2: one workgroup per row tile (rows over the bra / ket
+// pairs), the exchange blocks and the fixed pair's Coulomb block accumulated
+// in LDS as in eri_row_kernel and added to F once per tile.
+template <int LA, int LB, int LC, int LD, int MODE>
 __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba) {
   using S = Grp<LA, LB, LC, LD>;
   constexpr int G = S::G, NQ = S::NQ, NR = S::NR, NI = S::NI, NK = S::NK, NE = S::NE, NF = S::NF, NKC = S::NKC;
   constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND, NABCD = S::NABCD;
   constexpr int GSZ = NI * NK;
+  constexpr bool ROW = MODE != 0, TR = MODE == 2;
+  using R = RowShape<LA, LB, LC, LD, TR>;
   __shared__ double lds[NQ * S::SLOT];
   __shared__ int s_npq[NQ];
+  extern __shared__ double dyn[];  // row modes: jfix[NFIX], then rows[NROW][ncols]
 
   const int lane = threadIdx.x, slot = lane / G, gl = lane - slot * G;
   double *Gt = lds + slot * S::SLOT;
@@ -884,14 +1175,41 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
     }
   }
 
-  const int64_t nq = cd.q_end - cd.q_begin;
-  for (int64_t base = (int64_t)blockIdx.x * NQ; base < nq; base += (int64_t)gridDim.x * NQ) {
-    const int64_t q = cd.q_begin + base + slot;
+  // row modes: the tile, its fixed pair, the accumulators
+  int64_t qfirst = cd.q_begin, qlast = cd.q_end, qstep = (int64_t)gridDim.x * NQ, qoff = (int64_t)blockIdx.x * NQ;
+  int seg0 = 0, ncols = 0, off2 = 0;
+  int32_t ifix = 0;
+  double *jfix_s = dyn, *rows = dyn + R::NFIX;
+  if constexpr (ROW) {
+    const int tile = cd.t_begin + (int)blockIdx.x;
+    qfirst = cd.tile_q[tile] > cd.q_begin ? cd.tile_q[tile] : cd.q_begin;
+    qlast = cd.tile_q[tile + 1] < cd.q_end ? cd.tile_q[tile + 1] : cd.q_end;
+    qstep = NQ;
+    qoff = 0;
+    seg0 = cd.tile_seg[tile];
+    ifix = cd.seg_bra[seg0];
+    ncols = (int)row_columns<LA, LB, LC, LD, TR>(ba.ncol);
+    off2 = R::M1 == R::M2 ? 0 : ba.ncol[R::M1];
+    for (int i = lane; i < R::NFIX + R::NROW * ncols; i += 64) dyn[i] = 0.0;
+    __syncthreads();
+  }
+
+  const int64_t nq = qlast - qfirst;
+  for (int64_t base = qoff; base < nq; base += qstep) {
+    const int64_t q = qfirst + base + slot;
     const bool valid = base + slot < nq;
     int32_t ib = 0, ik = 0;
     PairRec bra{}, ket{};
     if (valid) {
-      decode_quartet(cd, q, ib, ik);
+      if constexpr (ROW) {
+        int sg = seg0;
+        while (sg + 1 < cd.nseg && cd.seg_start[sg + 1] <= q) ++sg;
+        const int32_t imov = cd.ket_list[cd.seg_ket0[sg] + (int32_t)(q - cd.seg_start[sg])];
+        ib = TR ? imov : ifix;
+        ik = TR ? ifix : imov;
+      } else {
+        decode_quartet(cd, q, ib, ik);
+      }
       bra = ba.pairs[ib];
       ket = ba.pairs[ik];
     }
@@ -1048,6 +1366,7 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
       for (int it = gl; it < S::DB; it += G) {
         double acc2 = 0.0, fac;
         int64_t fi, fj;
+        double *dst = nullptr;  // row modes: LDS accumulator of the element
         int k = it;
         if (k < S::NAB) {  // F[ab] += 4s sum_cd v D[cd]
           const int a = k / NB, b = k % NB;
@@ -1056,12 +1375,14 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
           fi = bra.bfA + gpos_of(LA, a);
           fj = bra.bfB + gpos_of(LB, b);
           fac = 4.0 * s;
+          if (MODE == 1) dst = &jfix_s[k];
         } else if ((k -= S::NAB) < S::NCD) {  // F[cd] += 4s sum_ab v D[ab]
 #pragma unroll
           for (int e = 0; e < NA * NB; ++e) acc2 += v[e * NC * ND + k] * DAB[e];
           fi = ket.bfA + gpos_of(LC, k / ND);
           fj = ket.bfB + gpos_of(LD, k % ND);
           fac = 4.0 * s;
+          if (MODE == 2) dst = &jfix_s[k];
         } else if ((k -= S::NCD) < S::NAC) {  // F[ac] -= s sum_bd v D[bd]
           const int a = k / NC, c = k % NC;
 #pragma unroll
@@ -1071,6 +1392,8 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
           fi = bra.bfA + gpos_of(LA, a);
           fj = ket.bfA + gpos_of(LC, c);
           fac = -s;
+          if (MODE == 1) dst = &rows[a * ncols + ket.cA + gpos_of(LC, c)];
+          if (MODE == 2) dst = &rows[c * ncols + bra.cA + gpos_of(LA, a)];
         } else if ((k -= S::NAC) < S::NBD) {  // F[bd] -= s sum_ac v D[ac]
           const int b = k / ND, d = k % ND;
 #pragma unroll
@@ -1080,6 +1403,8 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
           fi = bra.bfB + gpos_of(LB, b);
           fj = ket.bfB + gpos_of(LD, d);
           fac = -s;
+          if (MODE == 1) dst = &rows[(NA + b) * ncols + off2 + ket.cB + gpos_of(LD, d)];
+          if (MODE == 2) dst = &rows[(NC + d) * ncols + off2 + bra.cB + gpos_of(LB, b)];
         } else if ((k -= S::NBD) < S::NAD) {  // F[ad] -= s sum_bc v D[bc]
           const int a = k / ND, d = k % ND;
 #pragma unroll
@@ -1089,6 +1414,8 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
           fi = bra.bfA + gpos_of(LA, a);
           fj = ket.bfB + gpos_of(LD, d);
           fac = -s;
+          if (MODE == 1) dst = &rows[a * ncols + off2 + ket.cB + gpos_of(LD, d)];
+          if (MODE == 2) dst = &rows[(NC + d) * ncols + bra.cA + gpos_of(LA, a)];
         } else {  // F[bc] -= s sum_ad v D[ad]
           k -= S::NAD;
           const int b = k / NC, c = k % NC;
@@ -1099,11 +1426,35 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
           fi = bra.bfB + gpos_of(LB, b);
           fj = ket.bfA + gpos_of(LC, c);
           fac = -s;
+          if (MODE == 1) dst = &rows[(NA + b) * ncols + ket.cA + gpos_of(LC, c)];
+          if (MODE == 2) dst = &rows[c * ncols + off2 + bra.cB + gpos_of(LB, b)];
         }
-        if (acc2 != 0.0) fock_add(F, fi, fj, fac * acc2);
+        if (acc2 != 0.0) {
+          if (ROW && dst)
+            lds_add(dst, fac * acc2);
+          else
+            fock_add(F, fi, fj, fac * acc2);
+        }
       }
     }
     __syncthreads();  // slots are reused by the next round
+  }
+  if constexpr (ROW) {
+    // the tile's accumulators to F (the loop's last barrier ordered the adds)
+    const PairRec fix = ba.pairs[ifix];
+    for (int i = lane; i < R::NFIX; i += 64) {
+      const double x = jfix_s[i];
+      if (x != 0.0)
+        fock_add(ba.F, fix.bfA + gpos_of(R::L1, i / R::N2), fix.bfB + gpos_of(R::L2, i % R::N2), x);
+    }
+    for (int i = lane; i < R::NROW * ncols; i += 64) {
+      const double x = rows[i];
+      if (x == 0.0) continue;
+      const int r = i / ncols, k = i - r * ncols;
+      const int rbf = r < R::N1 ? fix.bfA + gpos_of(R::L1, r) : fix.bfB + gpos_of(R::L2, r - R::N1);
+      const int cbf = (R::M1 == R::M2 || k < off2) ? ba.col_bf[R::M1][k] : ba.col_bf[R::M2][k - off2];
+      fock_add(ba.F, rbf, cbf, x);
+    }
   }
 }
 
@@ -1145,6 +1496,168 @@ __global__ void segment_lengths_kernel(const double *bra_x, const int32_t *bra_g
   seg_len[id] = end > lo ? end - lo : 0;
 }
 
+// ------------------------------------------------------ Schwarz integrals
+// Schwarz value of a shell pair: sqrt(max over components |(ab|ab)|) of the
+// normalised cartesian components, the quantity the reference takes as
+// schwrz_int (src/interface/liberi_interface.F90:128).  One lane per pair;
+// the 2-D Rys integrals G(i, k) of each root (i on A, k on C = A) are moved
+// to the diagonal (a b | a b) in one step with the binomial form of the
+// transfer relation, I(a,b,a,b) = sum_t sum_u C(b,t) C(b,u) AB^(2b-t-u)
+// G(a+t, a+u), since the ket pair is the bra pair (CD = AB).
+__host__ __device__ constexpr int binom(int n, int k) {
+  int r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+  return r;
+}
+
+template <int LA, int LB, int NI>
+__device__ __forceinline__ void diag_transfer(const double (&g)[NI][NI], double ab, double (&h)[LA + 1][LB + 1]) {
+  double abp[2 * LB + 1];
+  abp[0] = 1.0;
+#pragma unroll
+  for (int k = 1; k <= 2 * LB; ++k) abp[k] = abp[k - 1] * ab;
+#pragma unroll
+  for (int a = 0; a <= LA; ++a)
+#pragma unroll
+    for (int b = 0; b <= LB; ++b) {
+      double s = 0.0;
+#pragma unroll
+      for (int t = 0; t <= b; ++t)
+#pragma unroll
+        for (int u = 0; u <= b; ++u) s += (double)(binom(b, t) * binom(b, u)) * abp[2 * b - t - u] * g[a + t][a + u];
+      h[a][b] = s;
+    }
+}
+
+template <int LA, int LB>
+__global__ __launch_bounds__(64) void schwarz_kernel(const PairRec *pairs, const PrimRec *prims, const int32_t *list,
+                                                     int32_t n, double xswitch, double *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  constexpr int L = LA + LB, NR = L + 1, NI = L + 1;
+  constexpr int NA = ncart(LA), NB = ncart(LB);
+  constexpr Range<LA, LA> ra{};
+  constexpr Range<LB, LB> rb{};
+  const PairRec pr = pairs[list[i]];
+  double acc[NA * NB];
+#pragma unroll
+  for (int k = 0; k < NA * NB; ++k) acc[k] = 0.0;
+  for (int ip = 0; ip < pr.nprim; ++ip) {
+    const PrimRec P = prims[pr.prim_off + ip];
+    for (int iq = 0; iq < pr.nprim; ++iq) {
+      const PrimRec Q = prims[pr.prim_off + iq];
+      const double pq = P.p + Q.p, ipq = 1.0 / pq;
+      const double pqx = P.Px - Q.Px, pqy = P.Py - Q.Py, pqz = P.Pz - Q.Pz;
+      const double X = P.p * Q.p * ipq * (pqx * pqx + pqy * pqy + pqz * pqz);
+      const double pref = kTwoPi52 * P.K * Q.K * sqrt(ipq);
+      double u[NR], w[NR];
+      rys_rule<NR>(X, xswitch, u, w);
+      const double hp = 0.5 / P.p, hq = 0.5 / Q.p;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const double t2 = u[r];
+        const double b00 = 0.5 * t2 * ipq;
+        const double b10 = hp - hp * Q.p * t2 * ipq;
+        const double b01 = hq - hq * P.p * t2 * ipq;
+        const double fa = Q.p * ipq * t2, fc = P.p * ipq * t2;
+        double h[3][LA + 1][LB + 1];
+        {
+          double g[NI][NI];
+          rys_2d<NI, NI>(g, P.PAx - fa * pqx, Q.PAx + fc * pqx, b10, b01, b00, 1.0);
+          diag_transfer<LA, LB, NI>(g, pr.ABx, h[0]);
+          rys_2d<NI, NI>(g, P.PAy - fa * pqy, Q.PAy + fc * pqy, b10, b01, b00, 1.0);
+          diag_transfer<LA, LB, NI>(g, pr.ABy, h[1]);
+          rys_2d<NI, NI>(g, P.PAz - fa * pqz, Q.PAz + fc * pqz, b10, b01, b00, w[r] * pref);
+          diag_transfer<LA, LB, NI>(g, pr.ABz, h[2]);
+        }
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+          for (int b = 0; b < NB; ++b)
+            acc[a * NB + b] += h[0][ra.lx[a]][rb.lx[b]] * h[1][ra.ly[a]][rb.ly[b]] * h[2][ra.lz[a]][rb.lz[b]];
+      }
+    }
+  }
+  double mx = 0.0;
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const double sc = cart_scale(ra.lx[a], ra.ly[a], ra.lz[a]) * cart_scale(rb.lx[b], rb.ly[b], rb.lz[b]);
+      const double v = fabs(acc[a * NB + b]) * sc * sc;
+      mx = v > mx ? v : mx;
+    }
+  out[i] = sqrt(mx);
+}
+
+template <int LA, int LB>
+static int launch_schwarz_t(const PairRec *pairs, const PrimRec *prims, const int32_t *list, int32_t n, double *out,
+                            hipStream_t st) {
+  const int tpb = 64;
+  hipLaunchKernelGGL((schwarz_kernel<LA, LB>), dim3((unsigned)((n + tpb - 1) / tpb)), dim3(tpb), 0, st, pairs,
+                     prims, list, n, rys_xmax_host[LA + LB + 1], out);
+  return hipGetLastError();
+}
+
+int launch_schwarz(int la, int lb, const PairRec *pairs, const PrimRec *prims, const int32_t *list, int32_t n,
+                   double *out, void *stream) {
+  if (n <= 0) return hipSuccess;
+  hipStream_t st = (hipStream_t)stream;
+  switch (la * 4 + lb) {
+    case 0: return launch_schwarz_t<0, 0>(pairs, prims, list, n, out, st);
+    case 4: return launch_schwarz_t<1, 0>(pairs, prims, list, n, out, st);
+    case 5: return launch_schwarz_t<1, 1>(pairs, prims, list, n, out, st);
+    case 8: return launch_schwarz_t<2, 0>(pairs, prims, list, n, out, st);
+    case 9: return launch_schwarz_t<2, 1>(pairs, prims, list, n, out, st);
+    case 10: return launch_schwarz_t<2, 2>(pairs, prims, list, n, out, st);
+    case 12: return launch_schwarz_t<3, 0>(pairs, prims, list, n, out, st);
+    case 13: return launch_schwarz_t<3, 1>(pairs, prims, list, n, out, st);
+    case 14: return launch_schwarz_t<3, 2>(pairs, prims, list, n, out, st);
+    case 15: return launch_schwarz_t<3, 3>(pairs, prims, list, n, out, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// F = 0 as a kernel (not a memset node) so that a build can be captured in a graph
+__global__ void fill_zero_kernel(double *p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+// max |p| over the array: block maximum, then one atomic max per block on the
+// bit pattern (order-preserving for non-negative doubles)
+__global__ void abs_max_kernel(const double *p, int64_t n, double *out) {
+  __shared__ double red[256];
+  double m = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    m = fmax(m, fabs(p[i]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    atomicMax((unsigned long long *)out, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+int launch_abs_max(const double *p, int64_t n, double *out, void *stream) {
+  if (n <= 0) return hipSuccess;
+  const int64_t need = (n + 255) / 256;
+  const unsigned g = (unsigned)(need < 1024 ? need : 1024);
+  hipLaunchKernelGGL(abs_max_kernel, dim3(g), dim3(256), 0, (hipStream_t)stream, p, n, out);
+  return hipGetLastError();
+}
+
+int launch_fill_zero(double *p, int64_t n, void *stream) {
+  if (n <= 0) return hipSuccess;
+  const int tpb = 256;
+  const int64_t need = (n + tpb - 1) / tpb;
+  const unsigned g = (unsigned)(need < 4096 ? need : 4096);
+  hipLaunchKernelGGL(fill_zero_kernel, dim3(g), dim3(tpb), 0, (hipStream_t)stream, p, n);
+  return hipGetLastError();
+}
+
 int launch_segment_lengths(const double *bra_x, const int32_t *bra_group, const int32_t *bra_pos, int32_t nbra,
                            const double *ket_x, const int32_t *gbeg, const int32_t *gend, int32_t ngroup,
                            int32_t diag, int32_t cmp, double cut, int64_t *seg_len, void *stream) {
@@ -1164,19 +1677,66 @@ constexpr bool use_small() {
   return nrange(LA, LA + LB) * nrange(LC, LC + LD) <= 100 && hrr_max_buffer(LA, LB, LC, LD) <= 100;
 }
 
+// LDS bytes of the row-tiled kernel of the class in orientation TR, or -1
+// when the class has no such kernel
+// (dynamic) LDS bytes of the row accumulators of the class in orientation TR
+template <int LA, int LB, int LC, int LD, bool TR>
+int64_t row_acc_bytes(const int32_t *ncol) {
+  using R = RowShape<LA, LB, LC, LD, TR>;
+  return (int64_t)sizeof(double) * (R::NFIX + R::NROW * row_columns<LA, LB, LC, LD, TR>(ncol));
+}
+
+template <int LA, int LB, int LC, int LD, bool TR>
+int64_t row_lds(const int32_t *ncol) {
+  using R = RowShape<LA, LB, LC, LD, TR>;
+  using GS = Grp<LA, LB, LC, LD>;
+  if constexpr (!R::exists)
+    return -1;
+  else if constexpr (use_small<LA, LB, LC, LD>())
+    return row_acc_bytes<LA, LB, LC, LD, TR>(ncol);
+  else if constexpr (GS::fits)  // plus the group kernel's static slots (and alignment slack)
+    return row_acc_bytes<LA, LB, LC, LD, TR>(ncol) + (int64_t)GS::NQ * GS::SLOT * 8 + GS::NQ * 4 + 256;
+  else
+    return -1;
+}
+
+template <int LA, int LB, int LC, int LD, bool TR>
+static int launch_row(const ClassDev &cd, const BuildArgs &ba, hipStream_t st) {
+  const int64_t lds = row_lds<LA, LB, LC, LD, TR>(ba.ncol);
+  if (lds < 0 || lds > kRowLdsBytes || cd.t_end <= cd.t_begin) return hipErrorInvalidValue;
+  const unsigned acc = (unsigned)row_acc_bytes<LA, LB, LC, LD, TR>(ba.ncol);
+  const dim3 grid((unsigned)(cd.t_end - cd.t_begin));
+  if constexpr (!RowShape<LA, LB, LC, LD, TR>::exists) {
+    return hipErrorInvalidValue;
+  } else if constexpr (use_small<LA, LB, LC, LD>()) {
+    hipLaunchKernelGGL((eri_row_kernel<LA, LB, LC, LD, TR>), grid, dim3(256), acc, st, cd, ba);
+  } else if constexpr (Grp<LA, LB, LC, LD>::fits) {
+    hipLaunchKernelGGL((eri_group_kernel<LA, LB, LC, LD, TR ? 2 : 1>), grid, dim3(64), acc, st, cd, ba);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 template <int LA, int LB, int LC, int LD>
 static int launch_one(const ClassDev &cd, const BuildArgs &ba, hipStream_t st, int nwg) {
   const int64_t n = cd.q_end - cd.q_begin;
   if (n <= 0) return hipSuccess;
   if constexpr (use_small<LA, LB, LC, LD>()) {
     const int tpb = 256;
-    hipLaunchKernelGGL((eri_small_kernel<LA, LB, LC, LD>), dim3((unsigned)((n + tpb - 1) / tpb)), dim3(tpb), 0,
-                       st, cd, ba);
+    if (cd.row_mode) {
+      return cd.transposed ? launch_row<LA, LB, LC, LD, true>(cd, ba, st) : launch_row<LA, LB, LC, LD, false>(cd, ba, st);
+    } else {
+      hipLaunchKernelGGL((eri_small_kernel<LA, LB, LC, LD>), dim3((unsigned)((n + tpb - 1) / tpb)), dim3(tpb), 0,
+                         st, cd, ba);
+    }
   } else if constexpr (Grp<LA, LB, LC, LD>::fits) {
+    if (cd.row_mode)
+      return cd.transposed ? launch_row<LA, LB, LC, LD, true>(cd, ba, st) : launch_row<LA, LB, LC, LD, false>(cd, ba, st);
     constexpr int NQ = Grp<LA, LB, LC, LD>::NQ;
     const int64_t need = (n + NQ - 1) / NQ;
     const int64_t g = need < kGroupWorkgroups ? need : kGroupWorkgroups;
-    hipLaunchKernelGGL((eri_group_kernel<LA, LB, LC, LD>), dim3((unsigned)g), dim3(64), 0, st, cd, ba);
+    hipLaunchKernelGGL((eri_group_kernel<LA, LB, LC, LD, 0>), dim3((unsigned)g), dim3(64), 0, st, cd, ba);
   } else {
     using S = Coop<LA, LB, LC, LD>;
     const int64_t g = n < nwg ? n : nwg;
@@ -1296,6 +1856,30 @@ int class_nroots(int c) {
 #undef X
     default:
       return 0;
+  }
+}
+
+int class_kernel_kind(int c) {
+  switch (c) {
+#define X(id, a, b, cc, d) \
+  case id:                 \
+    return use_small<a, b, cc, d>() ? 0 : (Grp<a, b, cc, d>::fits ? 1 : 2);
+    LIBERI_CLASS_LIST(X)
+#undef X
+    default:
+      return -1;
+  }
+}
+
+int64_t class_row_lds(int c, int transposed, const int32_t *ncol) {
+  switch (c) {
+#define X(id, a, b, cc, d)                                                   \
+  case id:                                                                   \
+    return transposed ? row_lds<a, b, cc, d, true>(ncol) : row_lds<a, b, cc, d, false>(ncol);
+    LIBERI_CLASS_LIST(X)
+#undef X
+    default:
+      return -1;
   }
 }
 

@@ -50,6 +50,9 @@ struct PairRec {
   double ABx, ABy, ABz;
   int32_t sameAB;    // A and B are one shell
   int32_t shA;       // shell index of A (0-based)
+  int32_t cA, cB;    // compact column of A's / B's first function among all
+                     // functions of that angular momentum (shell rank * ncart)
+  double x;          // Schwarz value of the pair
 };
 
 // One primitive pair (a in A, b in B): p = a + b, P, P - A, and
@@ -74,6 +77,17 @@ struct ClassDev {
   int32_t diag;
   int64_t q_begin, q_end;    // this rank's share of the class's quartets
   double xswitch;            // Rys argument from which the asymptotic rule is used
+  // Row tiles: runs of at most kTileQuartets quartets of one row.  Tile t
+  // covers [tile_q[t], tile_q[t+1]) and starts in segment tile_seg[t]; this
+  // rank's tiles are [t_begin, t_end).  row_mode selects the row-tiled kernel.
+  const int64_t *tile_q;     // ntile + 1
+  const int32_t *tile_seg;   // ntile
+  int32_t t_begin, t_end;
+  int32_t row_mode;
+  // 0: rows run over the class's bra pairs, segments over its kets (as
+  // above); 1: the roles are swapped -- seg_bra holds ket pairs and ket_list
+  // bra pairs -- so that a row tile fixes the ket pair (non-diagonal classes)
+  int32_t transposed;
 };
 
 struct BuildArgs {
@@ -84,6 +98,15 @@ struct BuildArgs {
   double value_cut;
   double *scratch;       // per-workgroup scratch for the cooperative kernels
   int64_t scratch_per_wg;  // doubles
+  int32_t nbf;           // basis functions
+  // compact columns of the row-tiled accumulators: the functions of angular
+  // momentum L in shell order, col_bf[L][k] = basis function of column k
+  const int32_t *col_bf[4];
+  int32_t ncol[4];
+  // mixed precision: quartets with x_bra * x_ket * dmax[0] < fp32_tau in FP32
+  double fp32_tau;
+  const double *dmax;        // max |D| of this build (device)
+  uint64_t *fp32_count;      // quartets evaluated in FP32 (device counter)
 };
 
 // Launch the kernel of class c over [q_begin, q_end).  Returns hipError_t.
@@ -93,10 +116,31 @@ int64_t class_scratch_doubles(int c);
 // Rys roots of class c and the end of its tabulated range
 int class_nroots(int c);
 double class_rys_xmax(int c);
+// LDS bytes of the row-tiled kernel of class c with rows over the bra pairs
+// (transposed = 0) or the ket pairs (1), given the compact column counts per
+// angular momentum; -1 when the class has no such kernel
+int64_t class_row_lds(int c, int transposed, const int32_t *ncol);
+// kernel shape of class c: 0 lane per quartet, 1 workgroup group per
+// quartet, 2 cooperative workgroup per quartet
+int class_kernel_kind(int c);
+// longest row tile, quartets
+constexpr int64_t kTileQuartets = 4096;
+// LDS the row-tiled kernel may take per workgroup
+constexpr int64_t kRowLdsBytes = 64 * 1024;
 // workgroups the cooperative kernels use
 constexpr int kCoopWorkgroups = 2048;
 // one-wave workgroups the group-per-quartet kernels use at most (persistent loop)
 constexpr int kGroupWorkgroups = 32768;
+
+// Schwarz values sqrt(max |(ab|ab)|) of the n shell pairs list[0..n) whose
+// shells have angular momenta la >= lb (A the la shell); out[i] for list[i].
+int launch_schwarz(int la, int lb, const PairRec *pairs, const PrimRec *prims, const int32_t *list, int32_t n,
+                   double *out, void *stream);
+
+// p[0..n) = 0 on the stream (a kernel, so it may sit in a captured graph)
+int launch_fill_zero(double *p, int64_t n, void *stream);
+// out[0] = max |p[0..n)| (out must be zero on entry)
+int launch_abs_max(const double *p, int64_t n, double *out, void *stream);
 
 // Segment lengths on the device: one thread per (kept bra row, ket group).
 int launch_segment_lengths(const double *bra_x, const int32_t *bra_group, const int32_t *bra_pos_in_group,

@@ -36,6 +36,8 @@ OPT_EXACT_LARGE_T = 4
 OPT_CLASS_TIMING = 5
 OPT_ONLY_CLASS = 6
 OPT_PRIM_CUTOFF = 7
+OPT_ROW_TILES = 8
+OPT_AUTOTUNE = 9
 
 _STATUS = {1: "EARG", 2: "ESTATE", 3: "ESIZE", 4: "EDEVICE", 5: "ENOMEM"}
 
@@ -50,7 +52,7 @@ class Stats(C.Structure):
     _fields_ = [("quartets", C.c_int64 * NCLASS), ("prim_quartets", C.c_int64 * NCLASS),
                 ("total_quartets", C.c_int64), ("fp32_quartets", C.c_int64), ("last_build_ms", C.c_double),
                 ("device_bytes", C.c_int64), ("class_ms", C.c_double * NCLASS), ("prim_pairs", C.c_int64),
-                ("prim_pairs_dropped", C.c_int64)]
+                ("prim_pairs_dropped", C.c_int64), ("class_variant", C.c_int32 * NCLASS)]
 
 
 _lib = None
@@ -81,8 +83,28 @@ def load_library(path: Path = LIB_PATH) -> C.CDLL:
     lib.liberi_c_init.argtypes = basis_args
     lib.liberi_c_fock.argtypes = [pdbl, pdbl, i64]
     lib.liberi_c_finalize.argtypes = []
+    lib.liberi_c_schwarz.argtypes = [i32, i32, i32, i32] + [pi32] * 5 + [pdbl] * 7
     _lib = lib
     return lib
+
+
+def schwarz(basis: Basis) -> np.ndarray:
+    """Schwarz integrals sqrt(max|(ab|ab)|) per shell pair, computed on the GPU,
+    in the layout of liberi_setup's schwrz_int (packed i(i-1)/2 + j, 1-based)."""
+    lib = load_library()
+    b = basis
+    keep, args = [], [b.nsh, b.natoms, b.num_bas, b.mxgtot]
+    for f in ("ang_mom", "contr_num", "sh_loc", "atom_num", "atom_loc"):
+        a, p = _i32(getattr(b, f))
+        keep.append(a)
+        args.append(p)
+    for f in ("exponents", "coef_s", "coef_p", "coef_d", "coef_f", "coords"):
+        a, p = _f64(getattr(b, f))
+        keep.append(a)
+        args.append(p)
+    out = np.zeros(b.nsh * (b.nsh + 1) // 2)
+    _check(lib.liberi_c_schwarz(*args, out.ctypes.data_as(C.POINTER(C.c_double))))
+    return out
 
 
 def _check(rc: int) -> None:
@@ -178,7 +200,8 @@ class Handle:
                 "total_quartets": s.total_quartets, "fp32_quartets": s.fp32_quartets,
                 "last_build_ms": s.last_build_ms, "device_bytes": s.device_bytes,
                 "class_ms": np.array(s.class_ms[:], np.float64), "prim_pairs": s.prim_pairs,
-                "prim_pairs_dropped": s.prim_pairs_dropped}
+                "prim_pairs_dropped": s.prim_pairs_dropped,
+                "class_variant": np.array(s.class_variant[:], dtype=np.int32)}
 
 
 # ---------------------------------------------- reference-named functions

@@ -54,6 +54,18 @@ int liberi_c_setup(liberi_ctx *ctx, int32_t nsh, int32_t natoms, int32_t num_bas
                    const double *contr_coef_p, const double *contr_coef_d, const double *contr_coef_f,
                    const double *coords, const double *schwrz_int, int32_t my_rank, int32_t num_procs);
 
+/* Schwarz integrals on the device: schwrz_int[i(i+1)/2 + j] (0-based shells
+ * i >= j) = sqrt(max over components |(ab|ab)|), the normalised-component
+ * convention of the schwrz_int argument of liberi_setup
+ * (src/interface/liberi_interface.F90:128), which the reference takes from its
+ * caller (GAMESS) and never computes.  Same basis arguments as
+ * liberi_c_setup; needs no handle.  Blocks until done. */
+int liberi_c_schwarz(int32_t nsh, int32_t natoms, int32_t num_bas, int32_t mxgtot, const int32_t *ang_mom,
+                     const int32_t *contr_num, const int32_t *sh_loc, const int32_t *atom_num,
+                     const int32_t *atom_loc, const double *exponents, const double *contr_coef_s,
+                     const double *contr_coef_p, const double *contr_coef_d, const double *contr_coef_f,
+                     const double *coords, double *schwrz_int);
+
 /* Replaces liberi_fock_build (src/interface/liberi_interface.F90:202-227) and
  * through it compute_integrals (src/liberi_driver.F90:52-167).  Host arrays of
  * n = num_bas(num_bas+1)/2 doubles; fock is overwritten.  Blocks until done. */
@@ -74,10 +86,13 @@ int liberi_c_cleanup(liberi_ctx *ctx);
 const char *liberi_c_last_error(void);
 
 /* Runtime knobs (compile-time parameters in the reference,
- * src/liberi_parameters.F90:38).  Take effect at the next setup. */
+ * src/liberi_parameters.F90:38).  Take effect at the next setup unless noted. */
 #define LIBERI_OPT_SCHWARZ_CUTOFF 1 /* quartet screening cutoff, default 5.00000000000000018e-11 */
 #define LIBERI_OPT_VALUE_CUTOFF 2   /* |integral| skip threshold, default 5.00000000000000018e-11 */
-#define LIBERI_OPT_FP32_TAU 3       /* mixed precision switch tau; 0 = plain FP64 (default) */
+#define LIBERI_OPT_FP32_TAU 3       /* mixed precision: quartets of the lane-per-quartet classes whose
+                                       Schwarz bound x_ab x_cd times max|D| is below tau are evaluated
+                                       in FP32 (Rys roots/weights and all Fock accumulation stay FP64);
+                                       0 = plain FP64 (default).  Read at each build. */
 #define LIBERI_OPT_EXACT_LARGE_T 4  /* 0 (default): large-argument forms from the reference's switch
                                        points (T > 25 for its Boys-based classes, X >= 37..65 by root
                                        count for its Rys classes); 1: only where exact to double */
@@ -85,6 +100,16 @@ const char *liberi_c_last_error(void);
 #define LIBERI_OPT_ONLY_CLASS 6     /* >= 0: build only this class (diagnostics); -1 (default): all */
 #define LIBERI_OPT_PRIM_CUTOFF 7    /* drop primitive pairs whose share of any integral is below this
                                        bound (default 1e-24; 0 keeps every primitive pair) */
+#define LIBERI_OPT_ROW_TILES 8      /* digestion: row tiles accumulate the exchange blocks on chip
+                                       per tile of quartets sharing one shell pair; direct digestion
+                                       adds every element of every quartet to F atomically.  1
+                                       (default): per class, see LIBERI_OPT_AUTOTUNE; 0: direct
+                                       everywhere; 2: row tiles wherever they fit.  Read at each
+                                       build. */
+#define LIBERI_OPT_AUTOTUNE 9       /* 1 (default): the first build after setup times direct and
+                                       row-tiled digestion per class and keeps the faster; 0: fixed
+                                       choice (row tiles for lane-per-quartet classes).  Setting it
+                                       makes the next build decide again. */
 int liberi_c_set_option(liberi_ctx *ctx, int32_t option, double value);
 int liberi_c_get_option(liberi_ctx *ctx, int32_t option, double *value);
 
@@ -101,6 +126,9 @@ typedef struct {
                                            LIBERI_OPT_CLASS_TIMING is on (else 0) */
   int64_t prim_pairs;                   /* primitive pairs kept on the device */
   int64_t prim_pairs_dropped;           /* primitive pairs removed by LIBERI_OPT_PRIM_CUTOFF */
+  int32_t class_variant[LIBERI_NCLASS]; /* digestion used per class: 0 direct atomics, 1 row tiles
+                                           over bra pairs, 2 row tiles over ket pairs; -1 class
+                                           inactive or not yet built */
 } liberi_stats;
 int liberi_c_get_stats(liberi_ctx *ctx, liberi_stats *out);
 

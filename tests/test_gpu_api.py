@@ -143,3 +143,51 @@ def test_exact_large_t_mode_matches_oracle(name, oracle):
     h.destroy()
     fo, _, _ = oracle.fock(case.basis, case.density)
     assert np.max(np.abs(f - fo)) <= 1e-11 * max(1.0, np.max(np.abs(fo)))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["cfg_ethane.npz", "cfg_cn_chain.npz"])
+def test_row_tiles_match_direct_digestion(name):
+    """Row-tiled digestion (exchange blocks of a tile's fixed pair accumulated
+    on chip, in either orientation) and per-quartet atomic digestion give the
+    same F up to summation order."""
+    case = Case.load_npz(GOLD / name)
+    h = liberi.Handle()
+    h.setup(case.basis)
+    assert h.get_option(liberi.OPT_ROW_TILES) == 1.0
+    h.set_option(liberi.OPT_ROW_TILES, 2)
+    tiled = h.fock_build(case.density)
+    variants = h.stats()["class_variant"]
+    h.set_option(liberi.OPT_ROW_TILES, 0)
+    direct = h.fock_build(case.density)
+    assert np.all(h.stats()["class_variant"][variants >= 0] == 0)
+    h.destroy()
+    assert np.any(variants == 1) and np.any(variants == 2)  # both orientations ran
+    scale = max(1.0, float(np.max(np.abs(direct))))
+    assert np.max(np.abs(tiled - direct)) <= 1e-11 * scale
+    assert np.max(np.abs(tiled - case.fock)) <= 1e-10
+
+
+@pytest.mark.gpu
+def test_autotuned_build_matches_reference():
+    """The first build times both digestion forms per class and keeps the
+    faster; every later build uses the choice.  F is the same either way."""
+    case = Case.load_npz(GOLD / "cfg_cn_chain.npz")
+    h = liberi.Handle()
+    h.setup(case.basis)
+    assert np.all(h.stats()["class_variant"] == -1)  # nothing chosen before the first build
+    f1 = h.fock_build(case.density)
+    v1 = h.stats()["class_variant"].copy()
+    f2 = h.fock_build(case.density)
+    v2 = h.stats()["class_variant"].copy()
+    h.set_option(liberi.OPT_AUTOTUNE, 0)
+    f3 = h.fock_build(case.density)
+    v3 = h.stats()["class_variant"].copy()
+    h.destroy()
+    active = v1 >= 0
+    assert active.sum() > 20 and np.array_equal(v1, v2)
+    assert np.array_equal(v3 >= 0, active)
+    for f in (f1, f2, f3):
+        assert np.max(np.abs(f - f1)) <= 1e-11
+    dE = 0.5 * float(np.dot(case.density, f1 - case.fock))
+    assert abs(dE) <= 1e-9

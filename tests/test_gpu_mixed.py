@@ -1,0 +1,77 @@
+"""Mixed-precision form: quartets whose Schwarz bound times max|D| is below
+tau are evaluated in FP32 (Rys roots/weights and Fock accumulation in FP64).
+
+Parity bar: the energy error against the FP64 build stays within the bound
+the switch implies and within 1e-6 Eh at tau = 1e-6; tau = 0 is the plain
+FP64 build.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from gms_liberi_amd import liberi
+from gms_liberi_amd.caseio import Case
+
+GOLD = Path(__file__).resolve().parent / "golden"
+ENERGY_TOL_TAU6 = 1e-6  # Eh, at tau = 1e-6
+
+
+def _energy(case, fock):
+    return 0.5 * float(np.dot(case.density, fock))
+
+
+@pytest.fixture(scope="module")
+def ethane():
+    return Case.load_npz(GOLD / "cfg_ethane.npz")
+
+
+@pytest.mark.gpu
+def test_tau_zero_is_fp64(ethane):
+    h = liberi.Handle()
+    h.setup(ethane.basis)
+    f0 = h.fock_build(ethane.density)
+    assert h.stats()["fp32_quartets"] == 0
+    h.set_option(liberi.OPT_FP32_TAU, 0.0)
+    f1 = h.fock_build(ethane.density)
+    h.destroy()
+    assert np.max(np.abs(f0 - f1)) <= 1e-12
+    assert np.max(np.abs(f0 - ethane.fock)) <= 1e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tau", [1e-4, 1e-5, 1e-6, 1e-7])
+def test_mixed_precision_error(ethane, tau):
+    h = liberi.Handle()
+    h.setup(ethane.basis)
+    ref = h.fock_build(ethane.density)
+    h.set_option(liberi.OPT_FP32_TAU, tau)
+    mix = h.fock_build(ethane.density)
+    nf32 = h.stats()["fp32_quartets"]
+    total = h.stats()["total_quartets"]
+    h.destroy()
+    assert 0 < nf32 <= total
+    dE = abs(_energy(ethane, mix) - _energy(ethane, ref))
+    dF = float(np.max(np.abs(mix - ref)))
+    # an FP32 quartet has |(ab|cd)| <= x_ab x_cd < tau / max|D| (Schwarz) and
+    # ~6e-8 relative error, so each moves an F element by < 6e-8 tau; even
+    # 1e6 such quartets on one element stay below tau
+    assert dF <= tau
+    if tau <= 1e-6:
+        assert dE <= ENERGY_TOL_TAU6
+    print(f"tau {tau:g}: fp32 quartets {nf32}/{total}, max|dF| {dF:.2e}, |dE| {dE:.2e}")
+
+
+@pytest.mark.gpu
+def test_mixed_precision_c60_energy():
+    case = Case.load_npz(GOLD / "cfg_c60.npz")
+    h = liberi.Handle()
+    h.setup(case.basis)
+    ref = h.fock_build(case.density)
+    h.set_option(liberi.OPT_FP32_TAU, 1e-6)
+    mix = h.fock_build(case.density)
+    nf32 = h.stats()["fp32_quartets"]
+    h.destroy()
+    dE = abs(_energy(case, mix) - _energy(case, ref))
+    assert nf32 > 0
+    assert dE <= ENERGY_TOL_TAU6
