@@ -44,21 +44,54 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def gpu_leg(case, steps: int, warmup: int):
+def profiled_traffic(workload: str, cls: str, variant: int):
+    """HBM bytes per launch of the class's kernel from the committed rocprof
+    counter summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), or
+    None when that kernel was not profiled."""
+    f = ROOT / "profiles" / f"fock_build_{workload}_traffic.txt"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        p = line.split()
+        if len(p) >= 4 and not line.startswith("#") and p[0] == cls and int(p[1]) == variant:
+            return {"fetch_bytes": int(p[2]), "write_bytes": int(p[3]), "bytes": int(p[2]) + int(p[3]),
+                    "source": f"profiles/{f.name} ({p[5] if len(p) > 5 else ''})"}
+    return None
+
+
+def timed_builds(h, D, F, n, stream, steps, hipmem):
+    start, end = hipmem.Event(), hipmem.Event()
+    stream.synchronize()
+    start.record(stream)
+    for _ in range(steps):
+        h.fock_build_device(D.address, F.address, n, stream.address)
+    end.record(stream)
+    end.synchronize()
+    return start.elapsed_ms(end) / steps
+
+
+def gpu_leg(case, steps: int, warmup: int, tau: float, graph: bool = True):
     from gms_liberi_amd import census, hipmem, liberi
 
     h = liberi.Handle()
     t0 = time.time()
     h.setup(case.basis)
     setup_s = time.time() - t0
+    h.set_option(liberi.OPT_GRAPH, 1 if graph else 0)
     n = case.basis.ntri
     D = hipmem.DeviceArray.from_host(case.density)
     F = hipmem.DeviceArray(n)
     stream = hipmem.Stream()
+    # the first build after setup also runs the per-class autotuner; it is part
+    # of setting up (like liberi_setup), never of the timed builds
+    t0 = time.time()
+    h.fock_build_device(D.address, F.address, n, stream.address)
+    stream.synchronize()
+    tune_s = time.time() - t0
     for _ in range(warmup):
         h.fock_build_device(D.address, F.address, n, stream.address)
     stream.synchronize()
-    log(f"warmup done ({warmup} builds), setup {setup_s:.3f}s")
+    log(f"setup {setup_s:.3f}s, first build with autotuning {tune_s:.3f}s, warmup done ({warmup} builds)")
 
     h.set_option(liberi.OPT_CLASS_TIMING, 1)
     class_ms = np.zeros(55)
@@ -74,9 +107,24 @@ def gpu_leg(case, steps: int, warmup: int):
     h.set_option(liberi.OPT_CLASS_TIMING, 0)
     st = h.stats()
     fock = F.to_host()
+    # plain FP64 without per-class events (the figure reported as value)
+    ms_plain = timed_builds(h, D, F, n, stream, steps, hipmem)
+    log(f"timed {steps} builds: {ms_plain:.3f} ms/build ({ms_per_step:.3f} with per-class events)")
+    mixed = None
+    if tau > 0:
+        h.set_option(liberi.OPT_FP32_TAU, tau)
+        h.fock_build_device(D.address, F.address, n, stream.address)
+        ms_mixed = timed_builds(h, D, F, n, stream, steps, hipmem)
+        fm = F.to_host()
+        nf32 = h.stats()["fp32_quartets"]
+        h.set_option(liberi.OPT_FP32_TAU, 0.0)
+        e64 = 0.5 * float(np.dot(case.density, fock))
+        mixed = dict(tau=tau, ms=ms_mixed, fp32_quartets=int(nf32), max_abs_dF=float(np.max(np.abs(fm - fock))),
+                     dE=0.5 * float(np.dot(case.density, fm)) - e64)
+        log(f"mixed tau={tau:g}: {ms_mixed:.3f} ms/build, {nf32} FP32 quartets, dE {mixed['dE']:.2e}")
     h.destroy()
     stream.destroy()
-    log(f"timed {steps} builds: {ms_per_step:.3f} ms/build")
+    ms_per_step = ms_plain
 
     class_ms /= steps
     q = st["quartets"].astype(np.float64)
@@ -90,7 +138,7 @@ def gpu_leg(case, steps: int, warmup: int):
                  for c in np.argsort(-class_ms)[:8] if class_ms[c] > 0}
     return dict(ms_per_step=ms_per_step, fock=fock, stats=st, dom=dom, achieved=achieved,
                 dom_ms=float(class_ms[dom]), dom_flops=float(flops[dom]), total_flops=total_flops,
-                per_class=per_class, setup_s=setup_s)
+                per_class=per_class, setup_s=setup_s, mixed=mixed)
 
 
 def sampled_case(case, keep: float, seed: int = 7):
@@ -144,6 +192,40 @@ def cpu_leg(case, keep: float):
     return dict(seconds=t, quartets=int(q_sample), threads=threads, pair_fraction=frac)
 
 
+def serving_leg(n: int, rate: float, seed: int, tau: float, slots: int):
+    """Request stream (Poisson arrivals of random H/C/N/O clusters, 10-200
+    atoms, 10 SCF-like iterations each), served in FP64 and in the mixed form;
+    per-request energy error of the mixed form against FP64."""
+    from gms_liberi_amd import serving
+
+    t0 = time.time()
+    reqs = serving.request_stream(n, rate, seed, iterations=10)
+    log(f"serving: {n} requests generated in {time.time() - t0:.1f}s "
+        f"(nbf {min(r.basis.num_bas for r in reqs)}..{max(r.basis.num_bas for r in reqs)})")
+    out = {}
+    focks = {}
+    for label, t in (("fp64", 0.0), ("mixed", tau)):
+        for r in reqs:
+            r.tau = t
+        srv = serving.Server(max_active=slots)
+        res = srv.run(reqs)
+        s = serving.summarize(res)
+        s["peak_active"] = srv.peak_active
+        out[label] = s
+        focks[label] = res
+        log(f"serving {label}: p50 {s['p50_latency_s']:.2f}s p99 {s['p99_latency_s']:.2f}s, "
+            f"{s['builds_per_s']:.2f} builds/s, peak {srv.peak_active} requests on the GPU")
+    dE = []
+    for r, a, b in zip(reqs, focks["fp64"], focks["mixed"]):
+        dE.append(max(abs(0.5 * float(np.dot(d, fb - fa))) for d, fa, fb in zip(r.densities, a.focks, b.focks)))
+    out["mixed"]["tau"] = tau
+    out["mixed"]["max_abs_dE_vs_fp64"] = max(dE)
+    out["workload"] = (f"{n} requests, Poisson arrivals at {rate}/s, random H/C/N/O clusters of 10-200 atoms "
+                       f"(s/p/d, nbf {min(r.basis.num_bas for r in reqs)}-{max(r.basis.num_bas for r in reqs)}), "
+                       f"10 SCF-like iterations each, up to {slots} requests on the GPU at once")
+    return out
+
+
 def copy_calibration(mib: int) -> None:
     """One device-to-device copy of `mib` MiB: the reference point that turns
     rocprof's FETCH_SIZE / WRITE_SIZE counters into HBM bytes."""
@@ -169,6 +251,13 @@ def main():
     ap.add_argument("--workload", default="c60", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-pairs", type=float, default=0.3, help="shell-pair fraction of the CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tau", type=float, default=1e-6, help="FP32 switch of the mixed-precision line (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=12, help="requests of the serving stream (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=1.0, help="arrivals per second of the serving stream")
+    ap.add_argument("--serve-slots", type=int, default=4, help="requests on the GPU at once")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the class kernels eagerly on one stream (profiling runs: kernel durations "
+                         "then match the per-class events)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (rocprof counter calibration)")
     a = ap.parse_args()
@@ -185,12 +274,13 @@ def main():
 
     case = Case.load_npz(ROOT / WORKLOADS[a.workload])
     log(f"workload {a.workload}: nbf={case.basis.num_bas} nsh={case.basis.nsh}")
-    g = gpu_leg(case, a.steps, a.warmup)
+    g = gpu_leg(case, a.steps, a.warmup, a.tau, graph=not a.no_graph)
     st = g["stats"]
     builds_per_s = 1000.0 / g["ms_per_step"]
     dF = float(np.max(np.abs(g["fock"] - case.fock)))
     dE = float(0.5 * np.dot(case.density, g["fock"] - case.fock))
 
+    serve = serving_leg(a.serve_requests, a.serve_rate, 5, a.tau, a.serve_slots) if a.serve_requests > 0 else None
     cpu = None if a.no_cpu else cpu_leg(case, a.cpu_pairs)
     cpu_base = None
     if cpu:
@@ -206,6 +296,7 @@ def main():
     from gms_liberi_amd import census
 
     dom = g["dom"]
+    traffic = profiled_traffic(a.workload, census.NAMES[dom], int(st["class_variant"][dom]))
     out = {
         "metric": METRIC,
         "value": round(builds_per_s, 4),
@@ -228,9 +319,17 @@ def main():
         "accuracy_vs_reference": {"max_abs_dF": dF, "dE_2e": dE},
         "roofline": {"bound": "valu", "kernel": census.NAMES[dom], "achieved": round(g["achieved"] * 1e-12, 3),
                      "peak": FP64_VALU_PEAK * 1e-12, "unit": "TFLOP/s",
-                     "frac": round(g["achieved"] / FP64_VALU_PEAK, 4), "traffic": None,
+                     "frac": round(g["achieved"] / FP64_VALU_PEAK, 4),
+                     "traffic": traffic["bytes"] if traffic else None,
+                     "traffic_detail": traffic,
                      "kernel_ms": round(g["dom_ms"], 4), "kernel_flops": g["dom_flops"]},
         "top_kernels": g["per_class"],
+        "mixed_precision": None if not g["mixed"] else {
+            "tau": g["mixed"]["tau"], "value": round(1000.0 / g["mixed"]["ms"], 4), "unit": "Fock builds/s",
+            "speedup_vs_fp64": round(g["ms_per_step"] / g["mixed"]["ms"], 3),
+            "fp32_quartets": g["mixed"]["fp32_quartets"], "max_abs_dF_vs_fp64": g["mixed"]["max_abs_dF"],
+            "dE_vs_fp64": g["mixed"]["dE"]},
+        "serving": serve,
         "cpu_baseline": cpu_base,
     }
     print(json.dumps(out), flush=True)

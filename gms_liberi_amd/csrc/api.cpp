@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -69,6 +70,8 @@ static int fail(int code, const std::string &msg) {
                   std::string(#expr) + ": " + hipGetErrorString(e_));                                   \
   } while (0)
 
+static void set_tuned_choice(int c, int v);
+
 // --------------------------------------------------------------- handle
 struct DevBuf {
   void *p = nullptr;
@@ -105,6 +108,15 @@ struct liberi_ctx {
   DevBuf aux;           // per build: max |D|, FP32 quartet count
   DevBuf tune_F;        // autotuner's scratch Fock matrix
   hipEvent_t tev[2] = {};
+  // batched form: the build's class launches spread over kSide streams and
+  // captured once into a graph, replayed by later builds on the same D and F
+  bool use_graph = true;
+  hipStream_t side[4] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[4] = {};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  const double *g_dD = nullptr;
+  double *g_dF = nullptr;
   bool fp32_pending = false;
   ClassState cls[kNumClasses];
   liberi_stats stats{};
@@ -167,7 +179,17 @@ static void dev_free(DevBuf &b) {
   b.bytes = 0;
 }
 
+static void drop_graph(liberi_ctx *h) {
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->gexec = nullptr;
+  h->graph = nullptr;
+  h->g_dD = nullptr;
+  h->g_dF = nullptr;
+}
+
 static void release_device(liberi_ctx *h) {
+  drop_graph(h);
   for (auto &c : h->cls) {
     dev_free(c.ket_list);
     dev_free(c.seg_start);
@@ -235,6 +257,11 @@ int liberi_c_destroy(liberi_ctx **ctx) {
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   for (hipEvent_t e : h->tev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : h->join_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
+  for (hipStream_t s2 : h->side)
+    if (s2) (void)hipStreamDestroy(s2);
   for (int c = 0; c < kNumClasses; ++c) {
     if (h->cev0[c]) (void)hipEventDestroy(h->cev0[c]);
     if (h->cev1[c]) (void)hipEventDestroy(h->cev1[c]);
@@ -247,6 +274,7 @@ int liberi_c_destroy(liberi_ctx **ctx) {
 
 int liberi_c_set_option(liberi_ctx *h, int32_t opt, double v) {
   if (!h) return fail(LIBERI_EARG, "liberi_c_set_option: NULL handle");
+  drop_graph(h);  // a captured build holds the options it was captured with
   if (opt == LIBERI_OPT_ONLY_CLASS) {
     if (!(v >= -1.0 && v < kNumClasses) || v != std::floor(v))
       return fail(LIBERI_EARG, "liberi_c_set_option: ONLY_CLASS must be -1 or a class index 0..54");
@@ -265,9 +293,11 @@ int liberi_c_set_option(liberi_ctx *h, int32_t opt, double v) {
       if (v != 0.0 && v != 1.0 && v != 2.0) return fail(LIBERI_EARG, "liberi_c_set_option: ROW_TILES must be 0, 1 or 2");
       h->row_tiles = (int)v;
       break;
+    case LIBERI_OPT_GRAPH: h->use_graph = v != 0.0; break;
     case LIBERI_OPT_AUTOTUNE:
       h->autotune = v != 0.0;
-      for (auto &c : h->cls) c.variant = -1;  // decide again at the next build
+      for (auto &c : h->cls) c.variant = -1;  // decide again at the next build,
+      set_tuned_choice(-1, -1);                // without earlier handles' choices
       break;
     default: return fail(LIBERI_EARG, "liberi_c_set_option: unknown option " + std::to_string(opt));
   }
@@ -286,6 +316,7 @@ int liberi_c_get_option(liberi_ctx *h, int32_t opt, double *v) {
     case LIBERI_OPT_ONLY_CLASS: *v = h->only_class; break;
     case LIBERI_OPT_ROW_TILES: *v = h->row_tiles; break;
     case LIBERI_OPT_AUTOTUNE: *v = h->autotune; break;
+    case LIBERI_OPT_GRAPH: *v = h->use_graph; break;
     default: return fail(LIBERI_EARG, "liberi_c_get_option: unknown option " + std::to_string(opt));
   }
   return LIBERI_OK;
@@ -686,6 +717,35 @@ int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, 
 // atomics), direct digestion for the workgroup classes (whose occupancy the
 // LDS accumulators would cut).
 constexpr int64_t kTuneMinQuartets = 20000;
+constexpr int kSideStreams = 4;
+
+// Choices made so far in this process, per class: later handles (a request
+// loop sets up one per molecule) reuse them instead of timing again.
+static std::mutex g_tune_mu;
+static int g_tuned[kNumClasses];
+static bool g_tuned_init = false;
+
+static int tuned_choice(int c) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (!g_tuned_init) {
+    for (int &v : g_tuned) v = -1;
+    g_tuned_init = true;
+  }
+  return g_tuned[c];
+}
+
+static void set_tuned_choice(int c, int v) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (!g_tuned_init) {
+    for (int &x : g_tuned) x = -1;
+    g_tuned_init = true;
+  }
+  if (c < 0) {
+    for (int &x : g_tuned) x = -1;
+  } else {
+    g_tuned[c] = v;
+  }
+}
 
 static int choose_variants(liberi_ctx *h, const BuildArgs &ba, hipStream_t st) {
   const int64_t ntri = (int64_t)h->num_bas * (h->num_bas + 1) / 2;
@@ -697,6 +757,10 @@ static int choose_variants(liberi_ctx *h, const BuildArgs &ba, hipStream_t st) {
     if (!h->autotune || h->row_tiles != 1 || !cs_.row_ok || nq < kTuneMinQuartets || cs_.dev.t_end <= cs_.dev.t_begin ||
         (h->only_class >= 0 && c != h->only_class)) {
       cs_.variant = dflt;
+      continue;
+    }
+    if (const int prev = tuned_choice(c); prev >= 0) {
+      cs_.variant = prev;
       continue;
     }
     if (!h->tune_F.p)
@@ -721,15 +785,71 @@ static int choose_variants(liberi_ctx *h, const BuildArgs &ba, hipStream_t st) {
       best[v] = std::min(best[v], ms);
     }
     cs_.variant = best[1] < best[0] ? 1 : 0;
+    set_tuned_choice(c, cs_.variant);
   }
   return LIBERI_OK;
 }
 
-static int build_on_device(liberi_ctx *h, const double *dD, double *dF, hipStream_t st) {
+// Zero F (and find max |D| for the mixed form), then every class's kernel.
+// With ns > 1 the classes are spread over the side streams, longest first
+// onto the least-loaded stream, forked from and joined back into st.
+static int enqueue_build(liberi_ctx *h, const BuildArgs &ba, const double *dD, double *dF, hipStream_t st, int ns) {
   const int64_t ntri = (int64_t)h->num_bas * (h->num_bas + 1) / 2;
   // fock = 0 (src/liberi_driver.F90:64)
   if (int e = launch_fill_zero(dF, ntri, st))
     return fail(LIBERI_EDEVICE, std::string("zero F: ") + hipGetErrorString((hipError_t)e));
+  if (h->fp32_tau > 0.0) {
+    int e = launch_fill_zero((double *)h->aux.p, 2, st);
+    if (e == hipSuccess) e = launch_abs_max(dD, ntri, (double *)h->aux.p, st);
+    if (e != hipSuccess) return fail(LIBERI_EDEVICE, std::string("max |D|: ") + hipGetErrorString((hipError_t)e));
+  }
+  std::vector<int> order;
+  for (int c = 0; c < kNumClasses; ++c)
+    if (h->cls[c].active && (h->only_class < 0 || c == h->only_class)) order.push_back(c);
+  std::vector<double> load(ns, 0.0);
+  // cost proxy: primitive quartets x Rys roots x sqrt(cartesian components of the class)
+  auto cost = [&](int c) {
+    const int la = kPairL[kClasses[c].bra][1], lb = kPairL[kClasses[c].bra][0];
+    const int lc = kPairL[kClasses[c].ket][1], ld = kPairL[kClasses[c].ket][0];
+    const double ncomp = (double)ncart_host(la) * ncart_host(lb) * ncart_host(lc) * ncart_host(ld);
+    return (double)h->cls[c].nprimq * class_nroots(c) * std::sqrt(ncomp);
+  };
+  if (ns > 1) {
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+    HIP_TRY(hipEventRecord(h->fork_ev, st));
+    for (int k = 0; k < ns; ++k) HIP_TRY(hipStreamWaitEvent(h->side[k], h->fork_ev, 0));
+  }
+  for (int c : order) {
+    hipStream_t cs = st;
+    if (ns > 1) {
+      // the cooperative kernels share one per-workgroup scratch buffer: their
+      // classes all go to side stream 0, which runs them one after another
+      const int k = class_scratch_doubles(c) > 0 ? 0 : (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      load[k] += cost(c);
+      cs = h->side[k];
+    }
+    if (h->class_timing && ns == 1) {
+      if (!h->cev0[c]) HIP_TRY(hipEventCreate(&h->cev0[c]));
+      if (!h->cev1[c]) HIP_TRY(hipEventCreate(&h->cev1[c]));
+      HIP_TRY(hipEventRecord(h->cev0[c], cs));
+    }
+    ClassDev cd = h->cls[c].dev;
+    cd.row_mode = h->cls[c].row_ok && cd.t_end > cd.t_begin &&
+                  (h->row_tiles == 2 || (h->row_tiles == 1 && h->cls[c].variant == 1));
+    int e = launch_class(c, cd, ba, cs, kCoopWorkgroups);
+    if (e != hipSuccess)
+      return fail(LIBERI_EDEVICE, std::string("launch ") + kClasses[c].name + ": " + hipGetErrorString((hipError_t)e));
+    if (h->class_timing && ns == 1) HIP_TRY(hipEventRecord(h->cev1[c], cs));
+  }
+  if (ns > 1)
+    for (int k = 0; k < ns; ++k) {
+      HIP_TRY(hipEventRecord(h->join_ev[k], h->side[k]));
+      HIP_TRY(hipStreamWaitEvent(st, h->join_ev[k], 0));
+    }
+  return LIBERI_OK;
+}
+
+static int build_on_device(liberi_ctx *h, const double *dD, double *dF, hipStream_t st) {
   BuildArgs ba;
   ba.pairs = (const PairRec *)h->pairs.p;
   ba.prims = (const PrimRec *)h->prims.p;
@@ -742,39 +862,48 @@ static int build_on_device(liberi_ctx *h, const double *dD, double *dF, hipStrea
   ba.fp32_tau = h->fp32_tau;
   ba.dmax = (const double *)h->aux.p;
   ba.fp32_count = (uint64_t *)h->aux.p + 1;
-  if (h->fp32_tau > 0.0) {
-    int e = launch_fill_zero((double *)h->aux.p, 2, st);
-    if (e == hipSuccess) e = launch_abs_max(dD, ntri, (double *)h->aux.p, st);
-    if (e != hipSuccess) return fail(LIBERI_EDEVICE, std::string("max |D|: ") + hipGetErrorString((hipError_t)e));
-  }
   for (int l = 0; l < 4; ++l) {
     ba.col_bf[l] = (const int32_t *)h->col_bf[l].p;
     ba.ncol[l] = h->ncol[l];
   }
   if (int rc = choose_variants(h, ba, st)) return rc;
-  HIP_TRY(hipEventRecord(h->ev0, st));
-  for (int c = 0; c < kNumClasses; ++c) {
-    if (!h->cls[c].active) continue;
-    if (h->only_class >= 0 && c != h->only_class) continue;
-    if (h->class_timing) {
-      if (!h->cev0[c]) HIP_TRY(hipEventCreate(&h->cev0[c]));
-      if (!h->cev1[c]) HIP_TRY(hipEventCreate(&h->cev1[c]));
-      HIP_TRY(hipEventRecord(h->cev0[c], st));
+  // the batched form: capture once (per D, F), replay afterwards.  Per-class
+  // timing and single-class diagnostics run eagerly on st.
+  const bool graph = h->use_graph && !h->class_timing && h->only_class < 0;
+  if (graph && (!h->gexec || h->g_dD != dD || h->g_dF != dF)) {
+    drop_graph(h);
+    for (int k = 0; k < kSideStreams; ++k) {
+      if (!h->side[k]) HIP_TRY(hipStreamCreateWithFlags(&h->side[k], hipStreamNonBlocking));
+      if (!h->join_ev[k]) HIP_TRY(hipEventCreateWithFlags(&h->join_ev[k], hipEventDisableTiming));
     }
-    ClassDev cd = h->cls[c].dev;
-    cd.row_mode = h->cls[c].row_ok && cd.t_end > cd.t_begin &&
-                  (h->row_tiles == 2 || (h->row_tiles == 1 && h->cls[c].variant == 1));
-    int e = launch_class(c, cd, ba, st, kCoopWorkgroups);
-    if (e != hipSuccess)
-      return fail(LIBERI_EDEVICE, std::string("launch ") + kClasses[c].name + ": " + hipGetErrorString((hipError_t)e));
-    if (h->class_timing) HIP_TRY(hipEventRecord(h->cev1[c], st));
+    if (!h->fork_ev) HIP_TRY(hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_build(h, ba, dD, dF, h->stream, kSideStreams);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(LIBERI_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+    h->graph = g;
+    HIP_TRY(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+    h->g_dD = dD;
+    h->g_dF = dF;
+  }
+  HIP_TRY(hipEventRecord(h->ev0, st));
+  if (graph) {
+    HIP_TRY(hipGraphLaunch(h->gexec, st));
+  } else if (int rc = enqueue_build(h, ba, dD, dF, st, 1)) {
+    return rc;
   }
   HIP_TRY(hipEventRecord(h->ev1, st));
   h->build_pending = true;
   h->fp32_pending = h->fp32_tau > 0.0;
   if (!h->fp32_pending) h->stats.fp32_quartets = 0;
-  h->timing_pending = h->class_timing;
-  if (!h->class_timing)
+  h->timing_pending = h->class_timing && !graph;
+  if (!h->timing_pending)
     for (double &v : h->stats.class_ms) v = 0.0;
   return LIBERI_OK;
 }

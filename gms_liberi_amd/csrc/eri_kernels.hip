@@ -1112,6 +1112,92 @@ struct GroupSize {
   static constexpr int G = (3 * NR > raw) ? 64 : raw;
 };
 
+__host__ __device__ constexpr int odd_at_least(int n) { return n | 1; }
+// longest row of the ket-phase arrays: NF at level 0, then each level's output
+__host__ __device__ constexpr int ket_row_max(int LC, int LD) {
+  int m = nrange(LC, LC + LD);
+  for (int j = 1; j <= LD; ++j) {
+    const int s = nrange(LC, LC + LD - j) * ncart(j);
+    m = s > m ? s : m;
+  }
+  return m;
+}
+__host__ __device__ constexpr int grp_buffer(int LA, int LB, int LC, int LD, int KS) {
+  const int NF = nrange(LC, LC + LD);
+  int m = ncart(LA) * ncart(LB) * KS;
+  if (LB > 0) {
+    const int s0 = nrange(LA, LA + LB) * NF;
+    m = s0 > m ? s0 : m;
+    for (int j = 1; j < LB; ++j) {
+      const int s = nrange(LA, LA + LB - j) * ncart(j) * NF;
+      m = s > m ? s : m;
+    }
+  }
+  return m;
+}
+
+// component k (internal order) of a shell of angular momentum L -> GAMESS
+// position and cartesian scale factor, without loops
+__device__ __forceinline__ int gpos_rt(int L, int k) {
+  if (L <= 1) return k;
+  if (L == 2) return (int)((0x251430u >> (4 * k)) & 15u);
+  return (int)((0x2861795430ull >> (4 * k)) & 15ull);
+}
+__device__ __forceinline__ double scale_rt(int L, int k) {
+  if (L == 2) return (k == 1 || k == 2 || k == 4) ? 1.7320508075688772935 : 1.0;
+  if (L == 3) return (k == 0 || k == 6 || k == 9) ? 1.0 : (k == 4 ? 3.8729833462074168852 : 2.2360679774997896964);
+  return 1.0;
+}
+
+// bra HRR level J for the columns f of this lane: out[a in range(LA, LAB-J)][b in
+// cart(J)][f] (row stride OS per (a, b)) from level J-1 (row stride NF)
+template <int LA, int LAB, int J, int NF, int G, int OS>
+__device__ __forceinline__ void grp_bra_level(const double *__restrict__ in, double *__restrict__ out, int gl,
+                                              double abx, double aby, double abz) {
+  constexpr Range<LA, LAB - J> ra{};
+  constexpr Range<J, J> rb{};
+  constexpr int nbm = ncart(J - 1), nb = ncart(J);
+  for (int f = gl; f < NF; f += G) {
+#pragma unroll
+    for (int a = 0; a < ra.n; ++a)
+#pragma unroll
+      for (int b = 0; b < nb; ++b) {
+        const int bx = rb.lx[b], by = rb.ly[b], bz = rb.lz[b];
+        const int dir = bx ? 0 : (by ? 1 : 2);
+        const int bmi = cidx(bx - (dir == 0), by - (dir == 1), bz - (dir == 2));
+        const int ax = ra.lx[a] + (dir == 0), ay = ra.ly[a] + (dir == 1), az = ra.lz[a] + (dir == 2);
+        const int api = nrange(LA, ra.l[a]) + cidx(ax, ay, az);
+        const double ab = dir == 0 ? abx : (dir == 1 ? aby : abz);
+        out[(a * nb + b) * OS + f] = in[(api * nbm + bmi) * NF + f] + ab * in[(a * nbm + bmi) * NF + f];
+      }
+  }
+}
+
+// ket HRR level J for the rows ab of this lane (row stride KS in and out)
+template <int LC, int LCD, int J, int NAB, int G, int KS>
+__device__ __forceinline__ void grp_ket_level(const double *__restrict__ in, double *__restrict__ out, int gl,
+                                              double cdx, double cdy, double cdz) {
+  constexpr Range<LC, LCD - J> rc{};
+  constexpr Range<J, J> rd{};
+  constexpr int ndm = ncart(J - 1), nd = ncart(J);
+  for (int ab = gl; ab < NAB; ab += G) {
+    const double *ip = in + ab * KS;
+    double *op = out + ab * KS;
+#pragma unroll
+    for (int c = 0; c < rc.n; ++c)
+#pragma unroll
+      for (int d = 0; d < nd; ++d) {
+        const int dx = rd.lx[d], dy = rd.ly[d], dz = rd.lz[d];
+        const int dir = dx ? 0 : (dy ? 1 : 2);
+        const int dmi = cidx(dx - (dir == 0), dy - (dir == 1), dz - (dir == 2));
+        const int cx = rc.lx[c] + (dir == 0), cy = rc.ly[c] + (dir == 1), cz = rc.lz[c] + (dir == 2);
+        const int cpi = nrange(LC, rc.l[c]) + cidx(cx, cy, cz);
+        const double cdv = dir == 0 ? cdx : (dir == 1 ? cdy : cdz);
+        op[c * nd + d] = ip[cpi * ndm + dmi] + cdv * ip[c * ndm + dmi];
+      }
+  }
+}
+
 template <int LA, int LB, int LC, int LD>
 struct Grp {
   static constexpr int G = GroupSize<LA, LB, LC, LD>::G;
@@ -1126,7 +1212,14 @@ struct Grp {
   static constexpr int BUF = hrr_max_buffer(LA, LB, LC, LD);
   static constexpr int NAB = NA * NB, NCD = NC * ND, NAC = NA * NC, NBD = NB * ND, NAD = NA * ND, NBC = NB * NC;
   static constexpr int DB = NAB + NCD + NAC + NBD + NAD + NBC;
-  static constexpr int SLOT = GS + 2 * BUF + DB;
+  // row stride of the ket-phase arrays [ab][...]: the longest ket-phase row,
+  // made odd so that lanes on consecutive rows hit distinct LDS banks
+  static constexpr int KS = odd_at_least(ket_row_max(LC, LD));
+  // buffers of the HRR: bra phase [e][b][f] (stride NF), ket phase [ab][KS]
+  static constexpr int BUFP = grp_buffer(LA, LB, LC, LD, KS);
+  // the G tables are dead once the primitive loop ends, the HRR buffers alias them
+  static constexpr int UNI = GS > 2 * BUFP ? GS : 2 * BUFP;
+  static constexpr int SLOT = UNI + DB;
   static constexpr int NKC = (NE * NF + G - 1) / G;
   static constexpr bool fits = SLOT * NQ * 8 <= 64 * 1024;
 };
@@ -1155,7 +1248,7 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
 
   const int lane = threadIdx.x, slot = lane / G, gl = lane - slot * G;
   double *Gt = lds + slot * S::SLOT;
-  double *buf0 = Gt + S::GS, *buf1 = buf0 + S::BUF, *Db = buf1 + S::BUF;
+  double *buf0 = Gt, *buf1 = buf0 + S::BUFP, *Db = Gt + S::UNI;
   double *DAB = Db, *DCD = DAB + S::NAB, *DAC = DCD + S::NCD, *DBD = DAC + S::NAC, *DAD = DBD + S::NBD,
          *DBC = DAD + S::NAD;
 
@@ -1266,94 +1359,78 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
       }
       __syncthreads();
     }
+    // (e0|f0) to LDS: rows e of stride NF for the bra HRR, or directly the
+    // ket-phase rows [a][KS] when there is no bra HRR
+    constexpr int ACCS = LB > 0 ? NF : S::KS;
 #pragma unroll
     for (int k = 0; k < NKC; ++k) {
       const int c = gl + G * k;
-      if (c < NE * NF) buf0[c] = acc[k];
+      if (c < NE * NF) {
+        const int e = c / NF;
+        buf0[e * ACCS + (c - e * NF)] = acc[k];
+      }
     }
     __syncthreads();
 
-    // bra HRR: level j out[a in range(LA, LAB-j)][b in cart(j)][f]
+    // bra HRR, lanes over the columns f; the last level writes rows [ab] of
+    // stride KS for the ket HRR, whose lanes run over the rows ab.  Component
+    // loops are compile-time, so every LDS address is a lane base plus a
+    // constant and the loads of one level issue back to back.
     double *in = buf0, *out = buf1;
-#pragma unroll
-    for (int j = 1; j <= LB; ++j) {
-      const int na = nrange(LA, S::LAB - j), nb = ncart(j), nbm = ncart(j - 1);
-      const int items = na * nb * NF;
-      for (int it = gl; it < items; it += G) {
-        const int f = it % NF, rest = it / NF, b = rest % nb, a = rest / nb;
-        int bx, by, bz, La, ax, ay, az;
-        cart_decode(j, b, bx, by, bz);
-        range_decode(LA, a, La, ax, ay, az);
-        const int dir = bx ? 0 : (by ? 1 : 2);
-        const int bmi = cidx(bx - (dir == 0), by - (dir == 1), bz - (dir == 2));
-        const int api = range_index(LA, ax + (dir == 0), ay + (dir == 1), az + (dir == 2));
-        const double abv = dir == 0 ? bra.ABx : (dir == 1 ? bra.ABy : bra.ABz);
-        out[(a * nb + b) * NF + f] = in[(api * nbm + bmi) * NF + f] + abv * in[(a * nbm + bmi) * NF + f];
-      }
+    if constexpr (LB >= 1) {
+      grp_bra_level<LA, S::LAB, 1, NF, G, (LB == 1 ? S::KS : NF)>(in, out, gl, bra.ABx, bra.ABy, bra.ABz);
       __syncthreads();
-      double *tmp = in;
-      in = out;
-      out = tmp;
+      double *t = in; in = out; out = t;
     }
-#pragma unroll
-    for (int j = 1; j <= LD; ++j) {
-      const int nc = nrange(LC, S::LCD - j), nd = ncart(j), ndm = ncart(j - 1);
-      const int nin = nrange(LC, S::LCD - j + 1) * ndm, nout = nc * nd;
-      const int items = NA * NB * nout;
-      for (int it = gl; it < items; it += G) {
-        const int ab = it / nout, rest = it - ab * nout, c = rest / nd, d = rest - c * nd;
-        int dx, dy, dz, Lc, cx, cy, cz;
-        cart_decode(j, d, dx, dy, dz);
-        range_decode(LC, c, Lc, cx, cy, cz);
-        const int dir = dx ? 0 : (dy ? 1 : 2);
-        const int dmi = cidx(dx - (dir == 0), dy - (dir == 1), dz - (dir == 2));
-        const int cpi = range_index(LC, cx + (dir == 0), cy + (dir == 1), cz + (dir == 2));
-        const double cdv = dir == 0 ? ket.ABx : (dir == 1 ? ket.ABy : ket.ABz);
-        out[ab * nout + c * nd + d] = in[ab * nin + cpi * ndm + dmi] + cdv * in[ab * nin + c * ndm + dmi];
-      }
+    if constexpr (LB >= 2) {
+      grp_bra_level<LA, S::LAB, 2, NF, G, (LB == 2 ? S::KS : NF)>(in, out, gl, bra.ABx, bra.ABy, bra.ABz);
       __syncthreads();
-      double *tmp = in;
-      in = out;
-      out = tmp;
+      double *t = in; in = out; out = t;
     }
-    // scale + value cutoff in place; gather the six density blocks
+    if constexpr (LB >= 3) {
+      grp_bra_level<LA, S::LAB, 3, NF, G, S::KS>(in, out, gl, bra.ABx, bra.ABy, bra.ABz);
+      __syncthreads();
+      double *t = in; in = out; out = t;
+    }
+    if constexpr (LD >= 1) {
+      grp_ket_level<LC, S::LCD, 1, S::NAB, G, S::KS>(in, out, gl, ket.ABx, ket.ABy, ket.ABz);
+      __syncthreads();
+      double *t = in; in = out; out = t;
+    }
+    if constexpr (LD >= 2) {
+      grp_ket_level<LC, S::LCD, 2, S::NAB, G, S::KS>(in, out, gl, ket.ABx, ket.ABy, ket.ABz);
+      __syncthreads();
+      double *t = in; in = out; out = t;
+    }
+    if constexpr (LD >= 3) {
+      grp_ket_level<LC, S::LCD, 3, S::NAB, G, S::KS>(in, out, gl, ket.ABx, ket.ABy, ket.ABz);
+      __syncthreads();
+      double *t = in; in = out; out = t;
+    }
+    constexpr int KS = S::KS, NCD = NC * ND;
+    // scale + value cutoff in place (rows ab of stride KS); gather the six density blocks
     if (valid) {
-      for (int it = gl; it < NABCD; it += G) {
-        const int d = it % ND, c = (it / ND) % NC, b = (it / (ND * NC)) % NB, a = it / (ND * NC * NB);
-        int x0, y0, z0, x1, y1, z1, x2, y2, z2, x3, y3, z3;
-        cart_decode(LA, a, x0, y0, z0);
-        cart_decode(LB, b, x1, y1, z1);
-        cart_decode(LC, c, x2, y2, z2);
-        cart_decode(LD, d, x3, y3, z3);
-        const double v = in[it] * cart_scale(x0, y0, z0) * cart_scale(x1, y1, z1) * cart_scale(x2, y2, z2) *
-                         cart_scale(x3, y3, z3);
-        in[it] = fabs(v) < ba.value_cut ? 0.0 : v;
+      constexpr Range<LC, LC> rc{};
+      constexpr Range<LD, LD> rd{};
+      for (int ab = gl; ab < S::NAB; ab += G) {
+        const double sab = scale_rt(LA, ab / NB) * scale_rt(LB, ab % NB);
+        double *row = in + ab * KS;
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+          for (int d = 0; d < ND; ++d) {
+            const double x = row[c * ND + d] * sab * (cart_scale(rc.lx[c], rc.ly[c], rc.lz[c]) *
+                                                      cart_scale(rd.lx[d], rd.ly[d], rd.lz[d]));
+            row[c * ND + d] = fabs(x) < ba.value_cut ? 0.0 : x;
+          }
       }
-      for (int it = gl; it < S::DB; it += G) {
-        int64_t i1, i2;
-        int k = it;
-        if (k < S::NAB) {
-          i1 = bra.bfA + gpos_of(LA, k / NB);
-          i2 = bra.bfB + gpos_of(LB, k % NB);
-        } else if ((k -= S::NAB) < S::NCD) {
-          i1 = ket.bfA + gpos_of(LC, k / ND);
-          i2 = ket.bfB + gpos_of(LD, k % ND);
-        } else if ((k -= S::NCD) < S::NAC) {
-          i1 = bra.bfA + gpos_of(LA, k / NC);
-          i2 = ket.bfA + gpos_of(LC, k % NC);
-        } else if ((k -= S::NAC) < S::NBD) {
-          i1 = bra.bfB + gpos_of(LB, k / ND);
-          i2 = ket.bfB + gpos_of(LD, k % ND);
-        } else if ((k -= S::NBD) < S::NAD) {
-          i1 = bra.bfA + gpos_of(LA, k / ND);
-          i2 = ket.bfB + gpos_of(LD, k % ND);
-        } else {
-          k -= S::NAD;
-          i1 = bra.bfB + gpos_of(LB, k / NC);
-          i2 = ket.bfA + gpos_of(LC, k % NC);
-        }
-        Db[it] = ba.D[tri(i1, i2)];
-      }
+      const double *Dm = ba.D;
+      for (int i = gl; i < S::NAB; i += G) DAB[i] = Dm[tri(bra.bfA + gpos_rt(LA, i / NB), bra.bfB + gpos_rt(LB, i % NB))];
+      for (int i = gl; i < NCD; i += G) DCD[i] = Dm[tri(ket.bfA + gpos_rt(LC, i / ND), ket.bfB + gpos_rt(LD, i % ND))];
+      for (int i = gl; i < S::NAC; i += G) DAC[i] = Dm[tri(bra.bfA + gpos_rt(LA, i / NC), ket.bfA + gpos_rt(LC, i % NC))];
+      for (int i = gl; i < S::NBD; i += G) DBD[i] = Dm[tri(bra.bfB + gpos_rt(LB, i / ND), ket.bfB + gpos_rt(LD, i % ND))];
+      for (int i = gl; i < S::NAD; i += G) DAD[i] = Dm[tri(bra.bfA + gpos_rt(LA, i / ND), ket.bfB + gpos_rt(LD, i % ND))];
+      for (int i = gl; i < S::NBC; i += G) DBC[i] = Dm[tri(bra.bfB + gpos_rt(LB, i / NC), ket.bfA + gpos_rt(LC, i % NC))];
     }
     __syncthreads();
     if (valid) {
@@ -1361,80 +1438,81 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
       if (bra.sameAB) s *= 0.5;
       if (ket.sameAB) s *= 0.5;
       if (cd.diag && ib == ik) s *= 0.5;
-      const double *v = in;
+      const double *__restrict__ v = in;
       double *F = ba.F;
-      for (int it = gl; it < S::DB; it += G) {
-        double acc2 = 0.0, fac;
-        int64_t fi, fj;
-        double *dst = nullptr;  // row modes: LDS accumulator of the element
-        int k = it;
-        if (k < S::NAB) {  // F[ab] += 4s sum_cd v D[cd]
-          const int a = k / NB, b = k % NB;
+      // adds x to F[i,j], or to its on-chip accumulator in the row modes
+      auto put = [&](int64_t fi, int64_t fj, double x, double *dst) {
+        if (x == 0.0) return;
+        if (ROW && dst)
+          lds_add(dst, x);
+        else
+          fock_add(F, fi, fj, x);
+      };
+      // J(AB): F[ab] += 4s sum_cd v D[cd]
+      for (int i = gl; i < S::NAB; i += G) {
+        const int a = i / NB, b = i - a * NB;
+        double t = 0.0;
 #pragma unroll
-          for (int e = 0; e < NC * ND; ++e) acc2 += v[(a * NB + b) * NC * ND + e] * DCD[e];
-          fi = bra.bfA + gpos_of(LA, a);
-          fj = bra.bfB + gpos_of(LB, b);
-          fac = 4.0 * s;
-          if (MODE == 1) dst = &jfix_s[k];
-        } else if ((k -= S::NAB) < S::NCD) {  // F[cd] += 4s sum_ab v D[ab]
+        for (int e = 0; e < NCD; ++e) t += v[i * KS + e] * DCD[e];
+        put(bra.bfA + gpos_rt(LA, a), bra.bfB + gpos_rt(LB, b), 4.0 * s * t, MODE == 1 ? &jfix_s[i] : nullptr);
+      }
+      // J(CD): F[cd] += 4s sum_ab v D[ab]
+      for (int i = gl; i < NCD; i += G) {
+        const int c = i / ND, d = i - c * ND;
+        double t = 0.0;
 #pragma unroll
-          for (int e = 0; e < NA * NB; ++e) acc2 += v[e * NC * ND + k] * DAB[e];
-          fi = ket.bfA + gpos_of(LC, k / ND);
-          fj = ket.bfB + gpos_of(LD, k % ND);
-          fac = 4.0 * s;
-          if (MODE == 2) dst = &jfix_s[k];
-        } else if ((k -= S::NCD) < S::NAC) {  // F[ac] -= s sum_bd v D[bd]
-          const int a = k / NC, c = k % NC;
+        for (int e = 0; e < S::NAB; ++e) t += v[e * KS + i] * DAB[e];
+        put(ket.bfA + gpos_rt(LC, c), ket.bfB + gpos_rt(LD, d), 4.0 * s * t, MODE == 2 ? &jfix_s[i] : nullptr);
+      }
+      // K(AC): F[ac] -= s sum_bd v D[bd]
+      for (int i = gl; i < S::NAC; i += G) {
+        const int a = i / NC, c = i - a * NC;
+        double t = 0.0;
 #pragma unroll
-          for (int b = 0; b < NB; ++b)
+        for (int b = 0; b < NB; ++b)
 #pragma unroll
-            for (int d = 0; d < ND; ++d) acc2 += v[((a * NB + b) * NC + c) * ND + d] * DBD[b * ND + d];
-          fi = bra.bfA + gpos_of(LA, a);
-          fj = ket.bfA + gpos_of(LC, c);
-          fac = -s;
-          if (MODE == 1) dst = &rows[a * ncols + ket.cA + gpos_of(LC, c)];
-          if (MODE == 2) dst = &rows[c * ncols + bra.cA + gpos_of(LA, a)];
-        } else if ((k -= S::NAC) < S::NBD) {  // F[bd] -= s sum_ac v D[ac]
-          const int b = k / ND, d = k % ND;
+          for (int d = 0; d < ND; ++d) t += v[(a * NB + b) * KS + c * ND + d] * DBD[b * ND + d];
+        const int pa = gpos_rt(LA, a), pc = gpos_rt(LC, c);
+        put(bra.bfA + pa, ket.bfA + pc, -s * t,
+            MODE == 1 ? &rows[a * ncols + ket.cA + pc] : MODE == 2 ? &rows[c * ncols + bra.cA + pa] : nullptr);
+      }
+      // K(BD): F[bd] -= s sum_ac v D[ac]
+      for (int i = gl; i < S::NBD; i += G) {
+        const int b = i / ND, d = i - b * ND;
+        double t = 0.0;
 #pragma unroll
-          for (int a = 0; a < NA; ++a)
+        for (int a = 0; a < NA; ++a)
 #pragma unroll
-            for (int c = 0; c < NC; ++c) acc2 += v[((a * NB + b) * NC + c) * ND + d] * DAC[a * NC + c];
-          fi = bra.bfB + gpos_of(LB, b);
-          fj = ket.bfB + gpos_of(LD, d);
-          fac = -s;
-          if (MODE == 1) dst = &rows[(NA + b) * ncols + off2 + ket.cB + gpos_of(LD, d)];
-          if (MODE == 2) dst = &rows[(NC + d) * ncols + off2 + bra.cB + gpos_of(LB, b)];
-        } else if ((k -= S::NBD) < S::NAD) {  // F[ad] -= s sum_bc v D[bc]
-          const int a = k / ND, d = k % ND;
+          for (int c = 0; c < NC; ++c) t += v[(a * NB + b) * KS + c * ND + d] * DAC[a * NC + c];
+        const int pb = gpos_rt(LB, b), pd = gpos_rt(LD, d);
+        put(bra.bfB + pb, ket.bfB + pd, -s * t,
+            MODE == 1   ? &rows[(NA + b) * ncols + off2 + ket.cB + pd]
+            : MODE == 2 ? &rows[(NC + d) * ncols + off2 + bra.cB + pb]
+                        : nullptr);
+      }
+      // K(AD): F[ad] -= s sum_bc v D[bc]
+      for (int i = gl; i < S::NAD; i += G) {
+        const int a = i / ND, d = i - a * ND;
+        double t = 0.0;
 #pragma unroll
-          for (int b = 0; b < NB; ++b)
+        for (int b = 0; b < NB; ++b)
 #pragma unroll
-            for (int c = 0; c < NC; ++c) acc2 += v[((a * NB + b) * NC + c) * ND + d] * DBC[b * NC + c];
-          fi = bra.bfA + gpos_of(LA, a);
-          fj = ket.bfB + gpos_of(LD, d);
-          fac = -s;
-          if (MODE == 1) dst = &rows[a * ncols + off2 + ket.cB + gpos_of(LD, d)];
-          if (MODE == 2) dst = &rows[(NC + d) * ncols + bra.cA + gpos_of(LA, a)];
-        } else {  // F[bc] -= s sum_ad v D[ad]
-          k -= S::NAD;
-          const int b = k / NC, c = k % NC;
+          for (int c = 0; c < NC; ++c) t += v[(a * NB + b) * KS + c * ND + d] * DBC[b * NC + c];
+        const int pa = gpos_rt(LA, a), pd = gpos_rt(LD, d);
+        put(bra.bfA + pa, ket.bfB + pd, -s * t,
+            MODE == 1 ? &rows[a * ncols + off2 + ket.cB + pd] : MODE == 2 ? &rows[(NC + d) * ncols + bra.cA + pa] : nullptr);
+      }
+      // K(BC): F[bc] -= s sum_ad v D[ad]
+      for (int i = gl; i < S::NBC; i += G) {
+        const int b = i / NC, c = i - b * NC;
+        double t = 0.0;
 #pragma unroll
-          for (int a = 0; a < NA; ++a)
+        for (int a = 0; a < NA; ++a)
 #pragma unroll
-            for (int d = 0; d < ND; ++d) acc2 += v[((a * NB + b) * NC + c) * ND + d] * DAD[a * ND + d];
-          fi = bra.bfB + gpos_of(LB, b);
-          fj = ket.bfA + gpos_of(LC, c);
-          fac = -s;
-          if (MODE == 1) dst = &rows[(NA + b) * ncols + ket.cA + gpos_of(LC, c)];
-          if (MODE == 2) dst = &rows[c * ncols + off2 + bra.cB + gpos_of(LB, b)];
-        }
-        if (acc2 != 0.0) {
-          if (ROW && dst)
-            lds_add(dst, fac * acc2);
-          else
-            fock_add(F, fi, fj, fac * acc2);
-        }
+          for (int d = 0; d < ND; ++d) t += v[(a * NB + b) * KS + c * ND + d] * DAD[a * ND + d];
+        const int pb = gpos_rt(LB, b), pc = gpos_rt(LC, c);
+        put(bra.bfB + pb, ket.bfA + pc, -s * t,
+            MODE == 1 ? &rows[(NA + b) * ncols + ket.cA + pc] : MODE == 2 ? &rows[c * ncols + off2 + bra.cB + pb] : nullptr);
       }
     }
     __syncthreads();  // slots are reused by the next round

@@ -32,6 +32,10 @@ def runtime() -> C.CDLL:
         h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
         h.hipEventSynchronize.argtypes = [C.c_void_p]
         h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        h.hipEventQuery.argtypes = [C.c_void_p]
+        h.hipHostMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_uint]
+        h.hipHostFree.argtypes = [C.c_void_p]
+        h.hipMemGetInfo.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
         h.hipGetErrorString.restype = C.c_char_p
         _hip = h
     return _hip
@@ -78,6 +82,43 @@ class DeviceArray:
             pass
 
 
+    def copy_from_async(self, host: "PinnedArray", stream: "Stream") -> None:
+        _ok(runtime().hipMemcpyAsync(self.ptr, host.ptr, self.n * 8, 1, stream.ptr), "hipMemcpyAsync H2D")
+
+    def copy_to_async(self, host: "PinnedArray", stream: "Stream") -> None:
+        _ok(runtime().hipMemcpyAsync(host.ptr, self.ptr, self.n * 8, 2, stream.ptr), "hipMemcpyAsync D2H")
+
+
+class PinnedArray:
+    """A float64 array in page-locked host memory (hipHostMalloc), for
+    asynchronous copies; .array is a numpy view of it."""
+
+    def __init__(self, n: int):
+        self.n = n
+        self.ptr = C.c_void_p()
+        _ok(runtime().hipHostMalloc(C.byref(self.ptr), max(1, n) * 8, 0), "hipHostMalloc")
+        self.array = np.ctypeslib.as_array(C.cast(self.ptr, C.POINTER(C.c_double)), shape=(n,))
+
+    def free(self) -> None:
+        if self.ptr:
+            self.array = None
+            runtime().hipHostFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_memory() -> tuple[int, int]:
+    """(free, total) bytes of HBM on the current device."""
+    free, total = C.c_size_t(), C.c_size_t()
+    _ok(runtime().hipMemGetInfo(C.byref(free), C.byref(total)), "hipMemGetInfo")
+    return free.value, total.value
+
+
 class Stream:
     def __init__(self):
         self.ptr = C.c_void_p()
@@ -106,6 +147,18 @@ class Event:
 
     def synchronize(self) -> None:
         _ok(runtime().hipEventSynchronize(self.ptr), "hipEventSynchronize")
+
+    def done(self) -> bool:
+        rc = runtime().hipEventQuery(self.ptr)
+        if rc == 600:  # hipErrorNotReady
+            return False
+        _ok(rc, "hipEventQuery")
+        return True
+
+    def destroy(self) -> None:
+        if self.ptr:
+            runtime().hipEventDestroy(self.ptr)
+            self.ptr = C.c_void_p()
 
     def elapsed_ms(self, end: "Event") -> float:
         ms = C.c_float()

@@ -38,6 +38,7 @@ OPT_ONLY_CLASS = 6
 OPT_PRIM_CUTOFF = 7
 OPT_ROW_TILES = 8
 OPT_AUTOTUNE = 9
+OPT_GRAPH = 10
 
 _STATUS = {1: "EARG", 2: "ESTATE", 3: "ESIZE", 4: "EDEVICE", 5: "ENOMEM"}
 

@@ -110,6 +110,12 @@ const char *liberi_c_last_error(void);
                                        row-tiled digestion per class and keeps the faster; 0: fixed
                                        choice (row tiles for lane-per-quartet classes).  Setting it
                                        makes the next build decide again. */
+#define LIBERI_OPT_GRAPH 10         /* 1 (default): batched form -- the build's class launches run on
+                                       four side streams (largest first onto the least-loaded
+                                       stream), captured once into a HIP graph that later builds on
+                                       the same D and F replay; 0: one stream, launched eagerly.
+                                       Per-class timing and ONLY_CLASS builds run eagerly.  Read at
+                                       each build; any option change drops the captured graph. */
 int liberi_c_set_option(liberi_ctx *ctx, int32_t option, double value);
 int liberi_c_get_option(liberi_ctx *ctx, int32_t option, double *value);
 

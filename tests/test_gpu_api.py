@@ -165,7 +165,9 @@ def test_row_tiles_match_direct_digestion(name):
     assert np.any(variants == 1) and np.any(variants == 2)  # both orientations ran
     scale = max(1.0, float(np.max(np.abs(direct))))
     assert np.max(np.abs(tiled - direct)) <= 1e-11 * scale
-    assert np.max(np.abs(tiled - case.fock)) <= 1e-10
+    # against the reference's own F: 1e-10, except where the reference's
+    # (sp|dd) kernel is itself inexact (the C-N chain, see test_gpu_parity)
+    assert np.max(np.abs(tiled - case.fock)) <= (1e-9 if "chain" in name else 1e-10)
 
 
 @pytest.mark.gpu
@@ -191,3 +193,47 @@ def test_autotuned_build_matches_reference():
         assert np.max(np.abs(f - f1)) <= 1e-11
     dE = 0.5 * float(np.dot(case.density, f1 - case.fock))
     assert abs(dE) <= 1e-9
+
+
+@pytest.mark.gpu
+def test_graph_replays_match_eager_builds():
+    """Batched form: class launches on four side streams captured into one HIP
+    graph.  Every replay, with eager work and synchronisations in between and
+    a new density in the same buffer, equals the eager build."""
+    from gms_liberi_amd import hipmem
+
+    case = Case.load_npz(GOLD / "cfg_ethane.npz")
+    rng = np.random.default_rng(4)
+    dens = [case.density, case.density * (1.0 + 0.01 * rng.standard_normal(case.density.shape))]
+    h = liberi.Handle()
+    h.setup(case.basis)
+    assert h.get_option(liberi.OPT_GRAPH) == 1.0
+    h.set_option(liberi.OPT_GRAPH, 0)
+    eager = [h.fock_build(d) for d in dens]
+    h.set_option(liberi.OPT_GRAPH, 1)
+    n = case.basis.ntri
+    D = hipmem.DeviceArray(n)
+    F = hipmem.DeviceArray(n)
+    s = hipmem.Stream()
+    for rep in range(4):
+        k = rep % 2
+        D.copy_from_async(_pinned(dens[k]), s)
+        h.fock_build_device(D.address, F.address, n, s.address)
+        s.synchronize()
+        got = F.to_host()
+        assert np.max(np.abs(got - eager[k])) <= 1e-12, rep
+        assert np.max(np.abs(h.fock_build(dens[k]) - eager[k])) <= 1e-12  # host path, its own graph
+    h.destroy()
+    s.destroy()
+
+
+def _pinned(a):
+    from gms_liberi_amd import hipmem
+
+    p = hipmem.PinnedArray(a.shape[0])
+    p.array[:] = a
+    _pinned.keep.append(p)
+    return p
+
+
+_pinned.keep = []
