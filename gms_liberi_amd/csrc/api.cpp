@@ -107,6 +107,7 @@ struct liberi_ctx {
   int32_t ncol[4] = {};
   DevBuf aux;           // per build: max |D|, FP32 quartet count
   DevBuf tune_F;        // autotuner's scratch Fock matrix
+  DevBuf dens_t, dens_b, fock_b;  // UHF: Da + Db, and the beta density / Fock of the host entry point
   hipEvent_t tev[2] = {};
   // batched form: the build's class launches spread over kSide streams and
   // captured once into a graph, replayed by later builds on the same D and F
@@ -206,6 +207,9 @@ static void release_device(liberi_ctx *h) {
   dev_free(h->scratch);
   dev_free(h->aux);
   dev_free(h->tune_F);
+  dev_free(h->dens_t);
+  dev_free(h->dens_b);
+  dev_free(h->fock_b);
   for (auto &b : h->col_bf) dev_free(b);
   for (auto &n : h->ncol) n = 0;
   h->stats = liberi_stats{};
@@ -849,11 +853,16 @@ static int enqueue_build(liberi_ctx *h, const BuildArgs &ba, const double *dD, d
   return LIBERI_OK;
 }
 
-static int build_on_device(liberi_ctx *h, const double *dD, double *dF, hipStream_t st) {
+// RHF: DK = D, k_scale = 1.  A UHF spin component: D = Da + Db, DK = that
+// spin's density, k_scale = 2 (F_s = J[Da+Db] - K[Ds]).
+static int build_on_device(liberi_ctx *h, const double *dD, double *dF, hipStream_t st, const double *dK = nullptr,
+                           double k_scale = 1.0) {
   BuildArgs ba;
   ba.pairs = (const PairRec *)h->pairs.p;
   ba.prims = (const PrimRec *)h->prims.p;
   ba.D = dD;
+  ba.DK = dK ? dK : dD;
+  ba.k_scale = k_scale;
   ba.F = dF;
   ba.value_cut = h->value_cut;
   ba.scratch = (double *)h->scratch.p;
@@ -869,7 +878,7 @@ static int build_on_device(liberi_ctx *h, const double *dD, double *dF, hipStrea
   if (int rc = choose_variants(h, ba, st)) return rc;
   // the batched form: capture once (per D, F), replay afterwards.  Per-class
   // timing and single-class diagnostics run eagerly on st.
-  const bool graph = h->use_graph && !h->class_timing && h->only_class < 0;
+  const bool graph = h->use_graph && !h->class_timing && h->only_class < 0 && ba.DK == dD && k_scale == 1.0;
   if (graph && (!h->gexec || h->g_dD != dD || h->g_dF != dF)) {
     drop_graph(h);
     for (int k = 0; k < kSideStreams; ++k) {
@@ -933,6 +942,49 @@ int liberi_c_fock_build_device(liberi_ctx *h, const double *dD, double *dF, int6
     return fail(LIBERI_ESIZE, "liberi_c_fock_build_device: n=" + std::to_string(n) + ", expected " + std::to_string(ntri));
   if (!dD || !dF) return fail(LIBERI_EARG, "liberi_c_fock_build_device: NULL array");
   return build_on_device(h, dD, dF, stream ? (hipStream_t)stream : h->stream);
+}
+
+// ------------------------------------------------------------------- UHF
+int liberi_c_fock_build_uhf_device(liberi_ctx *h, const double *dDa, const double *dDb, double *dFa, double *dFb,
+                                   int64_t n, void *stream) {
+  if (!h) return fail(LIBERI_EARG, "liberi_c_fock_build_uhf_device: NULL handle");
+  if (!h->initialized) return fail(LIBERI_ESTATE, "liberi_fock_build called before liberi_setup");
+  const int64_t ntri = (int64_t)h->num_bas * (h->num_bas + 1) / 2;
+  if (n != ntri)
+    return fail(LIBERI_ESIZE, "liberi_c_fock_build_uhf_device: n=" + std::to_string(n) + ", expected " + std::to_string(ntri));
+  if (!dDa || !dDb || !dFa || !dFb) return fail(LIBERI_EARG, "liberi_c_fock_build_uhf_device: NULL array");
+  hipStream_t st = stream ? (hipStream_t)stream : h->stream;
+  if (!h->dens_t.p)
+    if (int rc = dev_alloc(h, h->dens_t, ntri * sizeof(double))) return rc;
+  double *dt = (double *)h->dens_t.p;
+  if (int e = launch_add(dDa, dDb, dt, ntri, st))
+    return fail(LIBERI_EDEVICE, std::string("Da + Db: ") + hipGetErrorString((hipError_t)e));
+  if (int rc = build_on_device(h, dt, dFa, st, dDa, 2.0)) return rc;
+  return build_on_device(h, dt, dFb, st, dDb, 2.0);
+}
+
+int liberi_c_fock_build_uhf(liberi_ctx *h, const double *Da, const double *Db, double *Fa, double *Fb, int64_t n) {
+  if (!h) return fail(LIBERI_EARG, "liberi_c_fock_build_uhf: NULL handle");
+  if (!h->initialized) return fail(LIBERI_ESTATE, "liberi_fock_build called before liberi_setup");
+  const int64_t ntri = (int64_t)h->num_bas * (h->num_bas + 1) / 2;
+  if (n != ntri)
+    return fail(LIBERI_ESIZE, "liberi_c_fock_build_uhf: n=" + std::to_string(n) + ", expected " + std::to_string(ntri));
+  if (!Da || !Db || !Fa || !Fb) return fail(LIBERI_EARG, "liberi_c_fock_build_uhf: NULL array");
+  if (!h->dens_b.p)
+    if (int rc = dev_alloc(h, h->dens_b, ntri * sizeof(double))) return rc;
+  if (!h->fock_b.p)
+    if (int rc = dev_alloc(h, h->fock_b, ntri * sizeof(double))) return rc;
+  const size_t bytes = ntri * sizeof(double);
+  HIP_TRY(hipMemcpyAsync(h->dens.p, Da, bytes, hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipMemcpyAsync(h->dens_b.p, Db, bytes, hipMemcpyHostToDevice, h->stream));
+  if (int rc = liberi_c_fock_build_uhf_device(h, (const double *)h->dens.p, (const double *)h->dens_b.p,
+                                              (double *)h->fock.p, (double *)h->fock_b.p, n, h->stream))
+    return rc;
+  HIP_TRY(hipMemcpyAsync(Fa, h->fock.p, bytes, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipMemcpyAsync(Fb, h->fock_b.p, bytes, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  collect_timing(h);
+  return LIBERI_OK;
 }
 
 // ------------------------------------------- GAMESS direct-link entry points

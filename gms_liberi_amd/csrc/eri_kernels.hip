@@ -452,7 +452,7 @@ __global__ __launch_bounds__(256) void eri_small_kernel(ClassDev cd, BuildArgs b
 #pragma unroll
   for (int d = 0; d < ND; ++d) gd[d] = ket.bfB + gamess_pos(rd.lx[d], rd.ly[d], rd.lz[d]);
 
-  const double *Dm = ba.D;
+  const double *Dm = ba.D, *Dk = ba.DK;  // Coulomb and exchange densities
   double *F = ba.F;
   // J: F[ab] += 4s sum_cd v D[cd], F[cd] += 4s sum_ab v D[ab]
   {
@@ -485,17 +485,18 @@ __global__ __launch_bounds__(256) void eri_small_kernel(ClassDev cd, BuildArgs b
       if (t != 0.0) fock_add(F, gc[k / ND], gd[k % ND], 4.0 * s * t);
     }
   }
+  const double sk = s * ba.k_scale;
   // K: F[ac] -= s sum_bd v D[bd];  F[bd] -= s sum_ac v D[ac]
   {
     double dbd[NB * ND], dac[NA * NC];
 #pragma unroll
     for (int b = 0; b < NB; ++b)
 #pragma unroll
-      for (int d = 0; d < ND; ++d) dbd[b * ND + d] = Dm[tri(gb[b], gd[d])];
+      for (int d = 0; d < ND; ++d) dbd[b * ND + d] = Dk[tri(gb[b], gd[d])];
 #pragma unroll
     for (int a = 0; a < NA; ++a)
 #pragma unroll
-      for (int c = 0; c < NC; ++c) dac[a * NC + c] = Dm[tri(ga[a], gc[c])];
+      for (int c = 0; c < NC; ++c) dac[a * NC + c] = Dk[tri(ga[a], gc[c])];
 #pragma unroll
     for (int a = 0; a < NA; ++a)
 #pragma unroll
@@ -505,7 +506,7 @@ __global__ __launch_bounds__(256) void eri_small_kernel(ClassDev cd, BuildArgs b
         for (int b = 0; b < NB; ++b)
 #pragma unroll
           for (int d = 0; d < ND; ++d) t += res[((a * NB + b) * NC + c) * ND + d] * dbd[b * ND + d];
-        if (t != 0.0) fock_add(F, ga[a], gc[c], -s * t);
+        if (t != 0.0) fock_add(F, ga[a], gc[c], -sk * t);
       }
 #pragma unroll
     for (int b = 0; b < NB; ++b)
@@ -516,7 +517,7 @@ __global__ __launch_bounds__(256) void eri_small_kernel(ClassDev cd, BuildArgs b
         for (int a = 0; a < NA; ++a)
 #pragma unroll
           for (int c = 0; c < NC; ++c) t += res[((a * NB + b) * NC + c) * ND + d] * dac[a * NC + c];
-        if (t != 0.0) fock_add(F, gb[b], gd[d], -s * t);
+        if (t != 0.0) fock_add(F, gb[b], gd[d], -sk * t);
       }
   }
   // K: F[ad] -= s sum_bc v D[bc];  F[bc] -= s sum_ad v D[ad]
@@ -525,11 +526,11 @@ __global__ __launch_bounds__(256) void eri_small_kernel(ClassDev cd, BuildArgs b
 #pragma unroll
     for (int b = 0; b < NB; ++b)
 #pragma unroll
-      for (int c = 0; c < NC; ++c) dbc[b * NC + c] = Dm[tri(gb[b], gc[c])];
+      for (int c = 0; c < NC; ++c) dbc[b * NC + c] = Dk[tri(gb[b], gc[c])];
 #pragma unroll
     for (int a = 0; a < NA; ++a)
 #pragma unroll
-      for (int d = 0; d < ND; ++d) dad[a * ND + d] = Dm[tri(ga[a], gd[d])];
+      for (int d = 0; d < ND; ++d) dad[a * ND + d] = Dk[tri(ga[a], gd[d])];
 #pragma unroll
     for (int a = 0; a < NA; ++a)
 #pragma unroll
@@ -539,7 +540,7 @@ __global__ __launch_bounds__(256) void eri_small_kernel(ClassDev cd, BuildArgs b
         for (int b = 0; b < NB; ++b)
 #pragma unroll
           for (int c = 0; c < NC; ++c) t += res[((a * NB + b) * NC + c) * ND + d] * dbc[b * NC + c];
-        if (t != 0.0) fock_add(F, ga[a], gd[d], -s * t);
+        if (t != 0.0) fock_add(F, ga[a], gd[d], -sk * t);
       }
 #pragma unroll
     for (int b = 0; b < NB; ++b)
@@ -550,7 +551,7 @@ __global__ __launch_bounds__(256) void eri_small_kernel(ClassDev cd, BuildArgs b
         for (int a = 0; a < NA; ++a)
 #pragma unroll
           for (int d = 0; d < ND; ++d) t += res[((a * NB + b) * NC + c) * ND + d] * dad[a * ND + d];
-        if (t != 0.0) fock_add(F, gb[b], gc[c], -s * t);
+        if (t != 0.0) fock_add(F, gb[b], gc[c], -sk * t);
       }
   }
 }
@@ -617,7 +618,7 @@ __global__ __launch_bounds__(256) void eri_row_kernel(ClassDev cd, BuildArgs ba)
   const PairRec fix = ba.pairs[ifix];
   for (int i = threadIdx.x; i < NFIX + NROW * ncols; i += blockDim.x) lds[i] = 0.0;
 
-  const double *Dm = ba.D;
+  const double *Dm = ba.D, *Dk = ba.DK;  // Coulomb and exchange densities
   double *F = ba.F;
   // the fixed pair's density block and Coulomb partial sums
   double dfix[NFIX], jfix[NFIX];
@@ -708,17 +709,18 @@ __global__ __launch_bounds__(256) void eri_row_kernel(ClassDev cd, BuildArgs ba)
         if (t != 0.0) fock_add(F, ga[ab / NB], gb[ab % NB], 4.0 * s * t);
       }
     }
+    const double sk = s * ba.k_scale;
     // exchange: K(AC) and K(BD)
     {
       double dbd[NB * ND], dac[NA * NC];
 #pragma unroll
       for (int b = 0; b < NB; ++b)
 #pragma unroll
-        for (int d = 0; d < ND; ++d) dbd[b * ND + d] = Dm[tri(gb[b], gd[d])];
+        for (int d = 0; d < ND; ++d) dbd[b * ND + d] = Dk[tri(gb[b], gd[d])];
 #pragma unroll
       for (int a = 0; a < NA; ++a)
 #pragma unroll
-        for (int c = 0; c < NC; ++c) dac[a * NC + c] = Dm[tri(ga[a], gc[c])];
+        for (int c = 0; c < NC; ++c) dac[a * NC + c] = Dk[tri(ga[a], gc[c])];
 #pragma unroll
       for (int a = 0; a < NA; ++a)
 #pragma unroll
@@ -728,7 +730,7 @@ __global__ __launch_bounds__(256) void eri_row_kernel(ClassDev cd, BuildArgs ba)
           for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int d = 0; d < ND; ++d) t += v[((a * NB + b) * NC + c) * ND + d] * dbd[b * ND + d];
-          if (t != 0.0) lds_add(TR ? &rows[c * ncols + cm1[a]] : &rows[a * ncols + cm1[c]], -s * t);
+          if (t != 0.0) lds_add(TR ? &rows[c * ncols + cm1[a]] : &rows[a * ncols + cm1[c]], -sk * t);
         }
 #pragma unroll
       for (int b = 0; b < NB; ++b)
@@ -739,7 +741,7 @@ __global__ __launch_bounds__(256) void eri_row_kernel(ClassDev cd, BuildArgs ba)
           for (int a = 0; a < NA; ++a)
 #pragma unroll
             for (int c = 0; c < NC; ++c) t += v[((a * NB + b) * NC + c) * ND + d] * dac[a * NC + c];
-          if (t != 0.0) lds_add(TR ? &rows[(NC + d) * ncols + cm2[b]] : &rows[(NA + b) * ncols + cm2[d]], -s * t);
+          if (t != 0.0) lds_add(TR ? &rows[(NC + d) * ncols + cm2[b]] : &rows[(NA + b) * ncols + cm2[d]], -sk * t);
         }
     }
     // exchange: K(AD) and K(BC)
@@ -748,11 +750,11 @@ __global__ __launch_bounds__(256) void eri_row_kernel(ClassDev cd, BuildArgs ba)
 #pragma unroll
       for (int b = 0; b < NB; ++b)
 #pragma unroll
-        for (int c = 0; c < NC; ++c) dbc[b * NC + c] = Dm[tri(gb[b], gc[c])];
+        for (int c = 0; c < NC; ++c) dbc[b * NC + c] = Dk[tri(gb[b], gc[c])];
 #pragma unroll
       for (int a = 0; a < NA; ++a)
 #pragma unroll
-        for (int d = 0; d < ND; ++d) dad[a * ND + d] = Dm[tri(ga[a], gd[d])];
+        for (int d = 0; d < ND; ++d) dad[a * ND + d] = Dk[tri(ga[a], gd[d])];
 #pragma unroll
       for (int a = 0; a < NA; ++a)
 #pragma unroll
@@ -762,7 +764,7 @@ __global__ __launch_bounds__(256) void eri_row_kernel(ClassDev cd, BuildArgs ba)
           for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int c = 0; c < NC; ++c) t += v[((a * NB + b) * NC + c) * ND + d] * dbc[b * NC + c];
-          if (t != 0.0) lds_add(TR ? &rows[(NC + d) * ncols + cm1[a]] : &rows[a * ncols + cm2[d]], -s * t);
+          if (t != 0.0) lds_add(TR ? &rows[(NC + d) * ncols + cm1[a]] : &rows[a * ncols + cm2[d]], -sk * t);
         }
 #pragma unroll
       for (int b = 0; b < NB; ++b)
@@ -773,7 +775,7 @@ __global__ __launch_bounds__(256) void eri_row_kernel(ClassDev cd, BuildArgs ba)
           for (int a = 0; a < NA; ++a)
 #pragma unroll
             for (int d = 0; d < ND; ++d) t += v[((a * NB + b) * NC + c) * ND + d] * dad[a * ND + d];
-          if (t != 0.0) lds_add(TR ? &rows[c * ncols + cm2[b]] : &rows[(NA + b) * ncols + cm1[c]], -s * t);
+          if (t != 0.0) lds_add(TR ? &rows[c * ncols + cm2[b]] : &rows[(NA + b) * ncols + cm1[c]], -sk * t);
         }
     }
   }
@@ -992,8 +994,9 @@ __global__ __launch_bounds__((Coop<LA, LB, LC, LD>::NT)) void eri_coop_kernel(Cl
     if (ket.sameAB) s *= 0.5;
     if (cd.diag && ib == ik) s *= 0.5;
     const double *v = in;
-    const double *Dm = ba.D;
+    const double *Dm = ba.D, *Dk = ba.DK;
     double *F = ba.F;
+    const double ks = ba.k_scale;
     // six contracted blocks; one output element per lane iteration
     constexpr int n1 = NA * NB, n2 = NC * ND, n3 = NA * NC, n4 = NB * ND, n5 = NA * ND, n6 = NB * NC;
     for (int it = t; it < n1 + n2 + n3 + n4 + n5 + n6; it += NT) {
@@ -1042,10 +1045,10 @@ __global__ __launch_bounds__((Coop<LA, LB, LC, LD>::NT)) void eri_coop_kernel(Cl
           const int64_t gb = bra.bfB + gamess_pos(x, y, z);
           for (int d = 0; d < ND; ++d) {
             cart_decode(LD, d, x, y, z);
-            acc2 += v[((a * NB + b) * NC + c) * ND + d] * Dm[tri(gb, ket.bfB + gamess_pos(x, y, z))];
+            acc2 += v[((a * NB + b) * NC + c) * ND + d] * Dk[tri(gb, ket.bfB + gamess_pos(x, y, z))];
           }
         }
-        fac = -s;
+        fac = -s * ks;
       } else if (it < n1 + n2 + n3 + n4) {  // F[bd] -= s sum_ac v D[ac]
         const int k = it - n1 - n2 - n3, b = k / ND, d = k % ND;
         cart_decode(LB, b, x, y, z);
@@ -1057,10 +1060,10 @@ __global__ __launch_bounds__((Coop<LA, LB, LC, LD>::NT)) void eri_coop_kernel(Cl
           const int64_t ga = bra.bfA + gamess_pos(x, y, z);
           for (int c = 0; c < NC; ++c) {
             cart_decode(LC, c, x, y, z);
-            acc2 += v[((a * NB + b) * NC + c) * ND + d] * Dm[tri(ga, ket.bfA + gamess_pos(x, y, z))];
+            acc2 += v[((a * NB + b) * NC + c) * ND + d] * Dk[tri(ga, ket.bfA + gamess_pos(x, y, z))];
           }
         }
-        fac = -s;
+        fac = -s * ks;
       } else if (it < n1 + n2 + n3 + n4 + n5) {  // F[ad] -= s sum_bc v D[bc]
         const int k = it - n1 - n2 - n3 - n4, a = k / ND, d = k % ND;
         cart_decode(LA, a, x, y, z);
@@ -1072,10 +1075,10 @@ __global__ __launch_bounds__((Coop<LA, LB, LC, LD>::NT)) void eri_coop_kernel(Cl
           const int64_t gb = bra.bfB + gamess_pos(x, y, z);
           for (int c = 0; c < NC; ++c) {
             cart_decode(LC, c, x, y, z);
-            acc2 += v[((a * NB + b) * NC + c) * ND + d] * Dm[tri(gb, ket.bfA + gamess_pos(x, y, z))];
+            acc2 += v[((a * NB + b) * NC + c) * ND + d] * Dk[tri(gb, ket.bfA + gamess_pos(x, y, z))];
           }
         }
-        fac = -s;
+        fac = -s * ks;
       } else {  // F[bc] -= s sum_ad v D[ad]
         const int k = it - n1 - n2 - n3 - n4 - n5, b = k / NC, c = k % NC;
         cart_decode(LB, b, x, y, z);
@@ -1087,10 +1090,10 @@ __global__ __launch_bounds__((Coop<LA, LB, LC, LD>::NT)) void eri_coop_kernel(Cl
           const int64_t ga = bra.bfA + gamess_pos(x, y, z);
           for (int d = 0; d < ND; ++d) {
             cart_decode(LD, d, x, y, z);
-            acc2 += v[((a * NB + b) * NC + c) * ND + d] * Dm[tri(ga, ket.bfB + gamess_pos(x, y, z))];
+            acc2 += v[((a * NB + b) * NC + c) * ND + d] * Dk[tri(ga, ket.bfB + gamess_pos(x, y, z))];
           }
         }
-        fac = -s;
+        fac = -s * ks;
       }
       if (acc2 != 0.0) fock_add(F, fi, fj, fac * acc2);
     }
@@ -1424,13 +1427,13 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
             row[c * ND + d] = fabs(x) < ba.value_cut ? 0.0 : x;
           }
       }
-      const double *Dm = ba.D;
+      const double *Dm = ba.D, *Dk = ba.DK;  // Coulomb and exchange densities
       for (int i = gl; i < S::NAB; i += G) DAB[i] = Dm[tri(bra.bfA + gpos_rt(LA, i / NB), bra.bfB + gpos_rt(LB, i % NB))];
       for (int i = gl; i < NCD; i += G) DCD[i] = Dm[tri(ket.bfA + gpos_rt(LC, i / ND), ket.bfB + gpos_rt(LD, i % ND))];
-      for (int i = gl; i < S::NAC; i += G) DAC[i] = Dm[tri(bra.bfA + gpos_rt(LA, i / NC), ket.bfA + gpos_rt(LC, i % NC))];
-      for (int i = gl; i < S::NBD; i += G) DBD[i] = Dm[tri(bra.bfB + gpos_rt(LB, i / ND), ket.bfB + gpos_rt(LD, i % ND))];
-      for (int i = gl; i < S::NAD; i += G) DAD[i] = Dm[tri(bra.bfA + gpos_rt(LA, i / ND), ket.bfB + gpos_rt(LD, i % ND))];
-      for (int i = gl; i < S::NBC; i += G) DBC[i] = Dm[tri(bra.bfB + gpos_rt(LB, i / NC), ket.bfA + gpos_rt(LC, i % NC))];
+      for (int i = gl; i < S::NAC; i += G) DAC[i] = Dk[tri(bra.bfA + gpos_rt(LA, i / NC), ket.bfA + gpos_rt(LC, i % NC))];
+      for (int i = gl; i < S::NBD; i += G) DBD[i] = Dk[tri(bra.bfB + gpos_rt(LB, i / ND), ket.bfB + gpos_rt(LD, i % ND))];
+      for (int i = gl; i < S::NAD; i += G) DAD[i] = Dk[tri(bra.bfA + gpos_rt(LA, i / ND), ket.bfB + gpos_rt(LD, i % ND))];
+      for (int i = gl; i < S::NBC; i += G) DBC[i] = Dk[tri(bra.bfB + gpos_rt(LB, i / NC), ket.bfA + gpos_rt(LC, i % NC))];
     }
     __syncthreads();
     if (valid) {
@@ -1473,7 +1476,7 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
 #pragma unroll
           for (int d = 0; d < ND; ++d) t += v[(a * NB + b) * KS + c * ND + d] * DBD[b * ND + d];
         const int pa = gpos_rt(LA, a), pc = gpos_rt(LC, c);
-        put(bra.bfA + pa, ket.bfA + pc, -s * t,
+        put(bra.bfA + pa, ket.bfA + pc, -s * ba.k_scale * t,
             MODE == 1 ? &rows[a * ncols + ket.cA + pc] : MODE == 2 ? &rows[c * ncols + bra.cA + pa] : nullptr);
       }
       // K(BD): F[bd] -= s sum_ac v D[ac]
@@ -1485,7 +1488,7 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
 #pragma unroll
           for (int c = 0; c < NC; ++c) t += v[(a * NB + b) * KS + c * ND + d] * DAC[a * NC + c];
         const int pb = gpos_rt(LB, b), pd = gpos_rt(LD, d);
-        put(bra.bfB + pb, ket.bfB + pd, -s * t,
+        put(bra.bfB + pb, ket.bfB + pd, -s * ba.k_scale * t,
             MODE == 1   ? &rows[(NA + b) * ncols + off2 + ket.cB + pd]
             : MODE == 2 ? &rows[(NC + d) * ncols + off2 + bra.cB + pb]
                         : nullptr);
@@ -1499,7 +1502,7 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
 #pragma unroll
           for (int c = 0; c < NC; ++c) t += v[(a * NB + b) * KS + c * ND + d] * DBC[b * NC + c];
         const int pa = gpos_rt(LA, a), pd = gpos_rt(LD, d);
-        put(bra.bfA + pa, ket.bfB + pd, -s * t,
+        put(bra.bfA + pa, ket.bfB + pd, -s * ba.k_scale * t,
             MODE == 1 ? &rows[a * ncols + off2 + ket.cB + pd] : MODE == 2 ? &rows[(NC + d) * ncols + bra.cA + pa] : nullptr);
       }
       // K(BC): F[bc] -= s sum_ad v D[ad]
@@ -1511,7 +1514,7 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
 #pragma unroll
           for (int d = 0; d < ND; ++d) t += v[(a * NB + b) * KS + c * ND + d] * DAD[a * ND + d];
         const int pb = gpos_rt(LB, b), pc = gpos_rt(LC, c);
-        put(bra.bfB + pb, ket.bfA + pc, -s * t,
+        put(bra.bfB + pb, ket.bfA + pc, -s * ba.k_scale * t,
             MODE == 1 ? &rows[(NA + b) * ncols + ket.cA + pc] : MODE == 2 ? &rows[c * ncols + off2 + bra.cB + pb] : nullptr);
       }
     }
@@ -1724,6 +1727,19 @@ int launch_abs_max(const double *p, int64_t n, double *out, void *stream) {
   const int64_t need = (n + 255) / 256;
   const unsigned g = (unsigned)(need < 1024 ? need : 1024);
   hipLaunchKernelGGL(abs_max_kernel, dim3(g), dim3(256), 0, (hipStream_t)stream, p, n, out);
+  return hipGetLastError();
+}
+
+__global__ void add_kernel(const double *a, const double *b, double *out, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = a[i] + b[i];
+}
+
+int launch_add(const double *a, const double *b, double *out, int64_t n, void *stream) {
+  if (n <= 0) return hipSuccess;
+  const int64_t need = (n + 255) / 256;
+  const unsigned g = (unsigned)(need < 4096 ? need : 4096);
+  hipLaunchKernelGGL(add_kernel, dim3(g), dim3(256), 0, (hipStream_t)stream, a, b, out, n);
   return hipGetLastError();
 }
 

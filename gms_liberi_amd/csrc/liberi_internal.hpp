@@ -93,7 +93,9 @@ struct ClassDev {
 struct BuildArgs {
   const PairRec *pairs;
   const PrimRec *prims;
-  const double *D;
+  const double *D;   // density of the Coulomb blocks
+  const double *DK;  // density of the exchange blocks (= D for RHF)
+  double k_scale;    // exchange weight (1 for RHF: F = J[D] - K[D]/2; 2 for a UHF spin: J[Da+Db] - K[Ds])
   double *F;
   double value_cut;
   double *scratch;       // per-workgroup scratch for the cooperative kernels
@@ -139,6 +141,8 @@ int launch_schwarz(int la, int lb, const PairRec *pairs, const PrimRec *prims, c
 
 // p[0..n) = 0 on the stream (a kernel, so it may sit in a captured graph)
 int launch_fill_zero(double *p, int64_t n, void *stream);
+// out = a + b over n elements
+int launch_add(const double *a, const double *b, double *out, int64_t n, void *stream);
 // out[0] = max |p[0..n)| (out must be zero on entry)
 int launch_abs_max(const double *p, int64_t n, double *out, void *stream);
 

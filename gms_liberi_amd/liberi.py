@@ -85,6 +85,8 @@ def load_library(path: Path = LIB_PATH) -> C.CDLL:
     lib.liberi_c_fock.argtypes = [pdbl, pdbl, i64]
     lib.liberi_c_finalize.argtypes = []
     lib.liberi_c_schwarz.argtypes = [i32, i32, i32, i32] + [pi32] * 5 + [pdbl] * 7
+    lib.liberi_c_fock_build_uhf.argtypes = [vp, pdbl, pdbl, pdbl, pdbl, i64]
+    lib.liberi_c_fock_build_uhf_device.argtypes = [vp, vp, vp, vp, vp, i64, vp]
     _lib = lib
     return lib
 
@@ -163,6 +165,18 @@ class Handle:
             raise ValueError("fock must be a contiguous float64 array")
         _check(self._lib.liberi_c_fock_build(self._h, pd, fock.ctypes.data_as(C.POINTER(C.c_double)), d.shape[0]))
         return fock
+
+    def fock_build_uhf(self, density_a: np.ndarray, density_b: np.ndarray):
+        """Unrestricted Fock matrices (Fa, Fb) = S o (J[Da+Db] - K[Da]), S o (J[Da+Db] - K[Db])."""
+        da, pa = _f64(density_a)
+        db, pb = _f64(density_b)
+        if da.shape != db.shape:
+            raise ValueError("alpha and beta densities differ in size")
+        fa, fb = np.empty(da.shape[0]), np.empty(da.shape[0])
+        P = C.POINTER(C.c_double)
+        _check(self._lib.liberi_c_fock_build_uhf(self._h, pa, pb, fa.ctypes.data_as(P), fb.ctypes.data_as(P),
+                                                 da.shape[0]))
+        return fa, fb
 
     def fock_build_device(self, d_ptr: int, f_ptr: int, n: int, stream: int = 0) -> None:
         """Build on device-resident arrays (e.g. torch.Tensor.data_ptr()); asynchronous on `stream`."""

@@ -78,6 +78,18 @@ int liberi_c_fock_build(liberi_ctx *ctx, const double *density, double *fock, in
 int liberi_c_fock_build_device(liberi_ctx *ctx, const double *d_density, double *d_fock, int64_t n,
                                void *stream);
 
+/* Unrestricted (UHF) Fock matrices, the FOCK_B quantity the reference's MDI
+ * plugin registers but never computes (src/interface/liberi_mdi_api.F90:222-224,
+ * 404-409): Fa = S o (J[Da+Db] - K[Da]), Fb = S o (J[Da+Db] - K[Db]), with
+ * Da, Db the packed spin densities (same packing as the RHF density; Da = Db =
+ * D/2 gives Fa = Fb = the RHF F of D).  Two passes of the build (the integrals
+ * are evaluated twice).  Host arrays, blocking; the _device form takes HBM
+ * arrays and a hipStream_t and is asynchronous. */
+int liberi_c_fock_build_uhf(liberi_ctx *ctx, const double *density_a, const double *density_b, double *fock_a,
+                            double *fock_b, int64_t n);
+int liberi_c_fock_build_uhf_device(liberi_ctx *ctx, const double *d_density_a, const double *d_density_b,
+                                   double *d_fock_a, double *d_fock_b, int64_t n, void *stream);
+
 /* Replaces liberi_cleanup (src/interface/liberi_interface.F90:229-260):
  * releases the basis and all device data; the handle can be set up again. */
 int liberi_c_cleanup(liberi_ctx *ctx);
