@@ -567,13 +567,14 @@ int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, 
     const double sb = def.ps == PS_CROSS ? cut / mk : cut / mb;
     const double sk = def.ps == PS_CROSS ? cut / mb : cut / mk;
     const bool diag = def.bra == def.ket;
+    auto group_key = [&](int32_t p) { return (int64_t)pairs[p].nprim; };
     auto keep = [&](const std::vector<int32_t> &l, double thr) {
       std::vector<int32_t> out;
       for (int32_t p : l)
         if (pair_x[p] >= thr) out.push_back(p);
-      // sort by (primitive count, Schwarz value descending, pair index)
+      // sort by (group key, Schwarz value descending, pair index)
       std::sort(out.begin(), out.end(), [&](int32_t a, int32_t b) {
-        if (pairs[a].nprim != pairs[b].nprim) return pairs[a].nprim < pairs[b].nprim;
+        if (group_key(a) != group_key(b)) return group_key(a) < group_key(b);
         if (pair_x[a] != pair_x[b]) return pair_x[a] > pair_x[b];
         return a < b;
       });
@@ -597,10 +598,10 @@ int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, 
       cs_.row_ok = fit0 || fit1;
     }
     if (transposed) kb.swap(kk);
-    // ket groups (equal primitive count)
+    // ket groups (equal group key)
     std::vector<int32_t> gbeg, gend, kgroup(kk.size());
     for (size_t j = 0; j < kk.size(); ++j) {
-      if (j == 0 || pairs[kk[j]].nprim != pairs[kk[j - 1]].nprim) {
+      if (j == 0 || group_key(kk[j]) != group_key(kk[j - 1])) {
         if (j) gend.push_back((int32_t)j);
         gbeg.push_back((int32_t)j);
       }
@@ -642,6 +643,7 @@ int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, 
       return fail(LIBERI_EDEVICE, std::string("segment generation: ") + hipGetErrorString((hipError_t)e));
     std::vector<int64_t> sstart, tq;
     std::vector<int32_t> sbra, sket0, tseg;
+    const int64_t tile_len = kTileQuartets;
     int64_t total = 0, primq = 0;
     for (size_t i = 0; i < kb.size(); ++i) {
       const int64_t row0 = total;
@@ -655,9 +657,9 @@ int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, 
         total += L;
         primq += L * (int64_t)pairs[kb[i]].nprim * pairs[kk[gbeg[g]]].nprim;
       }
-      // row tiles: the row's quartets in runs of at most kTileQuartets
+      // row tiles: the row's quartets in runs of at most tile_len
       size_t sg = sfirst;
-      for (int64_t t0 = row0; t0 < total; t0 += kTileQuartets) {
+      for (int64_t t0 = row0; t0 < total; t0 += tile_len) {
         while (sg + 1 < sstart.size() && sstart[sg + 1] <= t0) ++sg;
         tq.push_back(t0);
         tseg.push_back((int32_t)sg);

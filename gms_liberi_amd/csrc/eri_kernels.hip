@@ -1220,8 +1220,9 @@ struct Grp {
   static constexpr int KS = odd_at_least(ket_row_max(LC, LD));
   // buffers of the HRR: bra phase [e][b][f] (stride NF), ket phase [ab][KS]
   static constexpr int BUFP = grp_buffer(LA, LB, LC, LD, KS);
-  // the G tables are dead once the primitive loop ends, the HRR buffers alias them
-  static constexpr int UNI = GS > 2 * BUFP ? GS : 2 * BUFP;
+  // two G-table buffers (primitive loop, double-buffered), dead once the loop
+  // ends: the HRR buffers alias them
+  static constexpr int UNI = 2 * GS > 2 * BUFP ? 2 * GS : 2 * BUFP;
   static constexpr int SLOT = UNI + DB;
   static constexpr int NKC = (NE * NF + G - 1) / G;
   static constexpr bool fits = SLOT * NQ * 8 <= 64 * 1024;
@@ -1233,21 +1234,19 @@ __device__ __forceinline__ int gpos_of(int L, int k) {
   return gamess_pos(x, y, z);
 }
 
-// MODE 0: persistent loop over the class's quartets, digestion by atomic adds
-// to F.  MODE 1 / 2: one workgroup per row tile (rows over the bra / ket
-// pairs), the exchange blocks and the fixed pair's Coulomb block accumulated
-// in LDS as in eri_row_kernel and added to F once per tile.
-template <int LA, int LB, int LC, int LD, int MODE>
+// Persistent loop over the class's quartets, NQ per one-wave workgroup and
+// round, digestion by atomic adds to F.  (Keeping exchange blocks on chip --
+// per row tile in LDS rows, or per run of kets sharing a shell -- cut this
+// kernel's atomics 5-9x on C60 but not its time: its atomics overlap its
+// arithmetic, and the LDS rows cost occupancy.)
+template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba) {
   using S = Grp<LA, LB, LC, LD>;
   constexpr int G = S::G, NQ = S::NQ, NR = S::NR, NI = S::NI, NK = S::NK, NE = S::NE, NF = S::NF, NKC = S::NKC;
   constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND, NABCD = S::NABCD;
   constexpr int GSZ = NI * NK;
-  constexpr bool ROW = MODE != 0, TR = MODE == 2;
-  using R = RowShape<LA, LB, LC, LD, TR>;
   __shared__ double lds[NQ * S::SLOT];
   __shared__ int s_npq[NQ];
-  extern __shared__ double dyn[];  // row modes: jfix[NFIX], then rows[NROW][ncols]
 
   const int lane = threadIdx.x, slot = lane / G, gl = lane - slot * G;
   double *Gt = lds + slot * S::SLOT;
@@ -1271,41 +1270,10 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
     }
   }
 
-  // row modes: the tile, its fixed pair, the accumulators
-  int64_t qfirst = cd.q_begin, qlast = cd.q_end, qstep = (int64_t)gridDim.x * NQ, qoff = (int64_t)blockIdx.x * NQ;
-  int seg0 = 0, ncols = 0, off2 = 0;
-  int32_t ifix = 0;
-  double *jfix_s = dyn, *rows = dyn + R::NFIX;
-  if constexpr (ROW) {
-    const int tile = cd.t_begin + (int)blockIdx.x;
-    qfirst = cd.tile_q[tile] > cd.q_begin ? cd.tile_q[tile] : cd.q_begin;
-    qlast = cd.tile_q[tile + 1] < cd.q_end ? cd.tile_q[tile + 1] : cd.q_end;
-    qstep = NQ;
-    qoff = 0;
-    seg0 = cd.tile_seg[tile];
-    ifix = cd.seg_bra[seg0];
-    ncols = (int)row_columns<LA, LB, LC, LD, TR>(ba.ncol);
-    off2 = R::M1 == R::M2 ? 0 : ba.ncol[R::M1];
-    for (int i = lane; i < R::NFIX + R::NROW * ncols; i += 64) dyn[i] = 0.0;
-    __syncthreads();
-  }
-
-  const int64_t nq = qlast - qfirst;
-  for (int64_t base = qoff; base < nq; base += qstep) {
-    const int64_t q = qfirst + base + slot;
-    const bool valid = base + slot < nq;
-    int32_t ib = 0, ik = 0;
+  // one round of NQ quartets (one per slot): integrals, HRR, digestion
+  auto round = [&](bool valid, int32_t ib, int32_t ik) {
     PairRec bra{}, ket{};
     if (valid) {
-      if constexpr (ROW) {
-        int sg = seg0;
-        while (sg + 1 < cd.nseg && cd.seg_start[sg + 1] <= q) ++sg;
-        const int32_t imov = cd.ket_list[cd.seg_ket0[sg] + (int32_t)(q - cd.seg_start[sg])];
-        ib = TR ? imov : ifix;
-        ik = TR ? ifix : imov;
-      } else {
-        decode_quartet(cd, q, ib, ik);
-      }
       bra = ba.pairs[ib];
       ket = ba.pairs[ik];
     }
@@ -1321,9 +1289,10 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
 #pragma unroll
     for (int k = 0; k < NKC; ++k) acc[k] = 0.0;
 
-    for (int t = 0; t < maxpq; ++t) {
-      const bool active = t < npq;
-      if (active && gl < 3 * NR) {
+    // 2-D Rys tables of primitive quartet t (3 NR lanes: one root and one
+    // direction each) into dst
+    auto tables = [&](int t, double *dst) {
+      if (t < npq && gl < 3 * NR) {
         const int ip = t / nk, iq = t - ip * nk;
         const PrimRec P = ba.prims[bra.prim_off + ip];
         const PrimRec Q = ba.prims[ket.prim_off + iq];
@@ -1347,16 +1316,25 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
 #pragma unroll
         for (int i = 0; i < NI; ++i)
 #pragma unroll
-          for (int k = 0; k < NK; ++k) Gt[(dim * NR + r) * GSZ + i * NK + k] = g[i][k];
+          for (int k = 0; k < NK; ++k) dst[(dim * NR + r) * GSZ + i * NK + k] = g[i][k];
       }
-      __syncthreads();
-      if (active) {
+    };
+    // software pipeline: while every lane accumulates quartet t from one
+    // buffer, the table lanes fill the other with quartet t+1 -- one barrier
+    // per primitive quartet
+    if (maxpq > 0) tables(0, Gt);
+    __syncthreads();
+    for (int t = 0; t < maxpq; ++t) {
+      const double *cur = (t & 1) ? Gt + S::GS : Gt;
+      double *nxt = (t & 1) ? Gt : Gt + S::GS;
+      if (t + 1 < maxpq) tables(t + 1, nxt);
+      if (t < npq) {
 #pragma unroll
         for (int k = 0; k < NKC; ++k) {
           double v = 0.0;
 #pragma unroll
           for (int r = 0; r < NR; ++r)
-            v += Gt[r * GSZ + offx[k]] * Gt[(NR + r) * GSZ + offy[k]] * Gt[(2 * NR + r) * GSZ + offz[k]];
+            v += cur[r * GSZ + offx[k]] * cur[(NR + r) * GSZ + offy[k]] * cur[(2 * NR + r) * GSZ + offz[k]];
           acc[k] += v;
         }
       }
@@ -1444,12 +1422,8 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
       const double *__restrict__ v = in;
       double *F = ba.F;
       // adds x to F[i,j], or to its on-chip accumulator in the row modes
-      auto put = [&](int64_t fi, int64_t fj, double x, double *dst) {
-        if (x == 0.0) return;
-        if (ROW && dst)
-          lds_add(dst, x);
-        else
-          fock_add(F, fi, fj, x);
+      auto put = [&](int64_t fi, int64_t fj, double x) {
+        if (x != 0.0) fock_add(F, fi, fj, x);
       };
       // J(AB): F[ab] += 4s sum_cd v D[cd]
       for (int i = gl; i < S::NAB; i += G) {
@@ -1457,7 +1431,7 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
         double t = 0.0;
 #pragma unroll
         for (int e = 0; e < NCD; ++e) t += v[i * KS + e] * DCD[e];
-        put(bra.bfA + gpos_rt(LA, a), bra.bfB + gpos_rt(LB, b), 4.0 * s * t, MODE == 1 ? &jfix_s[i] : nullptr);
+        put(bra.bfA + gpos_rt(LA, a), bra.bfB + gpos_rt(LB, b), 4.0 * s * t);
       }
       // J(CD): F[cd] += 4s sum_ab v D[ab]
       for (int i = gl; i < NCD; i += G) {
@@ -1465,7 +1439,7 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
         double t = 0.0;
 #pragma unroll
         for (int e = 0; e < S::NAB; ++e) t += v[e * KS + i] * DAB[e];
-        put(ket.bfA + gpos_rt(LC, c), ket.bfB + gpos_rt(LD, d), 4.0 * s * t, MODE == 2 ? &jfix_s[i] : nullptr);
+        put(ket.bfA + gpos_rt(LC, c), ket.bfB + gpos_rt(LD, d), 4.0 * s * t);
       }
       // K(AC): F[ac] -= s sum_bd v D[bd]
       for (int i = gl; i < S::NAC; i += G) {
@@ -1476,8 +1450,7 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
 #pragma unroll
           for (int d = 0; d < ND; ++d) t += v[(a * NB + b) * KS + c * ND + d] * DBD[b * ND + d];
         const int pa = gpos_rt(LA, a), pc = gpos_rt(LC, c);
-        put(bra.bfA + pa, ket.bfA + pc, -s * ba.k_scale * t,
-            MODE == 1 ? &rows[a * ncols + ket.cA + pc] : MODE == 2 ? &rows[c * ncols + bra.cA + pa] : nullptr);
+        put(bra.bfA + pa, ket.bfA + pc, -s * ba.k_scale * t);
       }
       // K(BD): F[bd] -= s sum_ac v D[ac]
       for (int i = gl; i < S::NBD; i += G) {
@@ -1488,10 +1461,7 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
 #pragma unroll
           for (int c = 0; c < NC; ++c) t += v[(a * NB + b) * KS + c * ND + d] * DAC[a * NC + c];
         const int pb = gpos_rt(LB, b), pd = gpos_rt(LD, d);
-        put(bra.bfB + pb, ket.bfB + pd, -s * ba.k_scale * t,
-            MODE == 1   ? &rows[(NA + b) * ncols + off2 + ket.cB + pd]
-            : MODE == 2 ? &rows[(NC + d) * ncols + off2 + bra.cB + pb]
-                        : nullptr);
+        put(bra.bfB + pb, ket.bfB + pd, -s * ba.k_scale * t);
       }
       // K(AD): F[ad] -= s sum_bc v D[bc]
       for (int i = gl; i < S::NAD; i += G) {
@@ -1502,8 +1472,7 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
 #pragma unroll
           for (int c = 0; c < NC; ++c) t += v[(a * NB + b) * KS + c * ND + d] * DBC[b * NC + c];
         const int pa = gpos_rt(LA, a), pd = gpos_rt(LD, d);
-        put(bra.bfA + pa, ket.bfB + pd, -s * ba.k_scale * t,
-            MODE == 1 ? &rows[a * ncols + off2 + ket.cB + pd] : MODE == 2 ? &rows[(NC + d) * ncols + bra.cA + pa] : nullptr);
+        put(bra.bfA + pa, ket.bfB + pd, -s * ba.k_scale * t);
       }
       // K(BC): F[bc] -= s sum_ad v D[ad]
       for (int i = gl; i < S::NBC; i += G) {
@@ -1514,28 +1483,18 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
 #pragma unroll
           for (int d = 0; d < ND; ++d) t += v[(a * NB + b) * KS + c * ND + d] * DAD[a * ND + d];
         const int pb = gpos_rt(LB, b), pc = gpos_rt(LC, c);
-        put(bra.bfB + pb, ket.bfA + pc, -s * ba.k_scale * t,
-            MODE == 1 ? &rows[(NA + b) * ncols + ket.cA + pc] : MODE == 2 ? &rows[c * ncols + off2 + bra.cB + pb] : nullptr);
+        put(bra.bfB + pb, ket.bfA + pc, -s * ba.k_scale * t);
       }
     }
     __syncthreads();  // slots are reused by the next round
-  }
-  if constexpr (ROW) {
-    // the tile's accumulators to F (the loop's last barrier ordered the adds)
-    const PairRec fix = ba.pairs[ifix];
-    for (int i = lane; i < R::NFIX; i += 64) {
-      const double x = jfix_s[i];
-      if (x != 0.0)
-        fock_add(ba.F, fix.bfA + gpos_of(R::L1, i / R::N2), fix.bfB + gpos_of(R::L2, i % R::N2), x);
-    }
-    for (int i = lane; i < R::NROW * ncols; i += 64) {
-      const double x = rows[i];
-      if (x == 0.0) continue;
-      const int r = i / ncols, k = i - r * ncols;
-      const int rbf = r < R::N1 ? fix.bfA + gpos_of(R::L1, r) : fix.bfB + gpos_of(R::L2, r - R::N1);
-      const int cbf = (R::M1 == R::M2 || k < off2) ? ba.col_bf[R::M1][k] : ba.col_bf[R::M2][k - off2];
-      fock_add(ba.F, rbf, cbf, x);
-    }
+  };
+
+  const int64_t nq = cd.q_end - cd.q_begin;
+  for (int64_t base = (int64_t)blockIdx.x * NQ; base < nq; base += (int64_t)gridDim.x * NQ) {
+    const bool valid = base + slot < nq;
+    int32_t ib = 0, ik = 0;
+    if (valid) decode_quartet(cd, cd.q_begin + base + slot, ib, ik);
+    round(valid, ib, ik);
   }
 }
 
@@ -1780,6 +1739,8 @@ int64_t row_acc_bytes(const int32_t *ncol) {
   return (int64_t)sizeof(double) * (R::NFIX + R::NROW * row_columns<LA, LB, LC, LD, TR>(ncol));
 }
 
+// LDS bytes of the row-tiled form of the class in orientation TR (lane
+// classes only), or -1 when the class has none
 template <int LA, int LB, int LC, int LD, bool TR>
 int64_t row_lds(const int32_t *ncol) {
   using R = RowShape<LA, LB, LC, LD, TR>;
@@ -1788,8 +1749,6 @@ int64_t row_lds(const int32_t *ncol) {
     return -1;
   else if constexpr (use_small<LA, LB, LC, LD>())
     return row_acc_bytes<LA, LB, LC, LD, TR>(ncol);
-  else if constexpr (GS::fits)  // plus the group kernel's static slots (and alignment slack)
-    return row_acc_bytes<LA, LB, LC, LD, TR>(ncol) + (int64_t)GS::NQ * GS::SLOT * 8 + GS::NQ * 4 + 256;
   else
     return -1;
 }
@@ -1804,8 +1763,6 @@ static int launch_row(const ClassDev &cd, const BuildArgs &ba, hipStream_t st) {
     return hipErrorInvalidValue;
   } else if constexpr (use_small<LA, LB, LC, LD>()) {
     hipLaunchKernelGGL((eri_row_kernel<LA, LB, LC, LD, TR>), grid, dim3(256), acc, st, cd, ba);
-  } else if constexpr (Grp<LA, LB, LC, LD>::fits) {
-    hipLaunchKernelGGL((eri_group_kernel<LA, LB, LC, LD, TR ? 2 : 1>), grid, dim3(64), acc, st, cd, ba);
   } else {
     return hipErrorInvalidValue;
   }
@@ -1825,12 +1782,11 @@ static int launch_one(const ClassDev &cd, const BuildArgs &ba, hipStream_t st, i
                          st, cd, ba);
     }
   } else if constexpr (Grp<LA, LB, LC, LD>::fits) {
-    if (cd.row_mode)
-      return cd.transposed ? launch_row<LA, LB, LC, LD, true>(cd, ba, st) : launch_row<LA, LB, LC, LD, false>(cd, ba, st);
+    if (cd.row_mode) return hipErrorInvalidValue;
     constexpr int NQ = Grp<LA, LB, LC, LD>::NQ;
     const int64_t need = (n + NQ - 1) / NQ;
     const int64_t g = need < kGroupWorkgroups ? need : kGroupWorkgroups;
-    hipLaunchKernelGGL((eri_group_kernel<LA, LB, LC, LD, 0>), dim3((unsigned)g), dim3(64), 0, st, cd, ba);
+    hipLaunchKernelGGL((eri_group_kernel<LA, LB, LC, LD>), dim3((unsigned)g), dim3(64), 0, st, cd, ba);
   } else {
     using S = Coop<LA, LB, LC, LD>;
     const int64_t g = n < nwg ? n : nwg;
