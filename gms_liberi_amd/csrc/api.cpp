@@ -592,7 +592,8 @@ int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, 
     {
       const int nbra = ncart_host(kPairL[def.bra][0]) * ncart_host(kPairL[def.bra][1]);
       const int nket = ncart_host(kPairL[def.ket][0]) * ncart_host(kPairL[def.ket][1]);
-      const int64_t l0 = class_row_lds(c, 0, h->ncol), l1 = diag ? -1 : class_row_lds(c, 1, h->ncol);
+      const double xsw = h->exact_large_t ? class_rys_xmax(c) : reference_switch(c);
+      const int64_t l0 = class_row_lds(c, 0, h->ncol, xsw), l1 = diag ? -1 : class_row_lds(c, 1, h->ncol, xsw);
       const bool fit0 = l0 >= 0 && l0 <= kRowLdsBytes, fit1 = l1 >= 0 && l1 <= kRowLdsBytes;
       transposed = fit1 && (!fit0 || nket > nbra);
       cs_.row_ok = fit0 || fit1;

@@ -102,12 +102,15 @@ __device__ __forceinline__ int64_t tri(int64_t i, int64_t j) {
 // never exceeds rys_xmax[N]; the asymptotic rule reproduces exactly the
 // large-argument Boys values F_m = (2m-1)!!/(2X)^m sqrt(pi/4X) that the
 // reference uses past its switch points.
+// tab: the N-root part of the table (rys_table + rys_offset[N], or a copy of
+// its first intervals in LDS)
 template <int N>
-__device__ __forceinline__ void rys_rule(double X, double xswitch, double (&u)[N], double (&w)[N]) {
+__device__ __forceinline__ void rys_rule(double X, double xswitch, double (&u)[N], double (&w)[N],
+                                         const double *tab = rys_table + rys_offset[N]) {
   if (X < xswitch) {
     const int j = (int)(X * RYS_INV_H);
     const double s = X * (2.0 * RYS_INV_H) - (double)(2 * j + 1);
-    const double *c = rys_table + rys_offset[N] + j * (2 * N * (RYS_DEG + 1));
+    const double *c = tab + j * (2 * N * (RYS_DEG + 1));
 #pragma unroll
     for (int r = 0; r < 2 * N; ++r) {
       const double *cr = c + r * (RYS_DEG + 1);
@@ -326,7 +329,8 @@ __device__ __forceinline__ bool mixed_low(const BuildArgs &ba, const PairRec &br
 template <class T, int LA, int LB, int LC, int LD>
 __device__ __forceinline__ void small_integrals(const PairRec &bra, const PairRec &ket, const PrimRec *prims,
                                                 double xswitch, double cut,
-                                                double (&v)[Small<LA, LB, LC, LD>::NABCD]) {
+                                                double (&v)[Small<LA, LB, LC, LD>::NABCD],
+                                                const double *tab = rys_table + rys_offset[Small<LA, LB, LC, LD>::NR]) {
   using S = Small<LA, LB, LC, LD>;
   constexpr int NE = S::NE, NF = S::NF, NR = S::NR, NI = S::LAB + 1, NK = S::LCD + 1;
   constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND;
@@ -348,7 +352,7 @@ __device__ __forceinline__ void small_integrals(const PairRec &bra, const PairRe
       const double X = rho * (pqx * pqx + pqy * pqy + pqz * pqz);
       const double pref = kTwoPi52 * P.K * Q.K * sqrt(ipq);
       double u[NR], w[NR];
-      rys_rule<NR>(X, xswitch, u, w);
+      rys_rule<NR>(X, xswitch, u, w, tab);
       const T tp = (T)P.p, tq = (T)Q.p, tipq = (T)ipq;
       const T hp = (T)0.5 / tp, hq = (T)0.5 / tq;
       const T tpqx = (T)pqx, tpqy = (T)pqy, tpqz = (T)pqz;
@@ -562,6 +566,16 @@ __device__ __forceinline__ void lds_add(double *p, double v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Doubles of the N-root table that the rule reads below xswitch (its first
+// ceil(xswitch / h) intervals).
+__host__ __device__ inline int64_t rys_table_used(int n, double xswitch) {
+  int64_t nint = (int64_t)ceil(xswitch * RYS_INV_H);
+  const int64_t cap = n == 1 ? 78 : n == 2 ? 93 : n == 3 ? 107 : n == 4 ? 120 : n == 5 ? 132 : n == 6 ? 144 : 155;
+  if (nint > cap) nint = cap;
+  if (nint < 0) nint = 0;
+  return nint * 2 * n * (RYS_DEG + 1);
+}
+
 // Shape of a row tile of class (LA LB | LC LD).  The tile fixes one shell
 // pair (the bra pair, or with TR the ket pair): its shells give the rows of
 // the on-chip Fock accumulators.  The other ("moving") pair varies per
@@ -605,10 +619,17 @@ __global__ __launch_bounds__(256) void eri_row_kernel(ClassDev cd, BuildArgs ba)
   constexpr Range<R::L2, R::L2> r2{};
   constexpr Range<R::M1, R::M1> rm1{};
   constexpr Range<R::M2, R::M2> rm2{};
-  extern __shared__ double lds[];  // jfix[NFIX], then rows[NROW][ncols]
+  extern __shared__ double lds[];  // jfix[NFIX], rows[NROW][ncols], then the Rys table
   double *jfix_s = lds, *rows = lds + NFIX;
   const int ncols = (int)row_columns<LA, LB, LC, LD, TR>(ba.ncol);
   const int off2 = R::M1 == R::M2 ? 0 : ba.ncol[R::M1];
+  // the class's Rys table below the switch, staged once per workgroup
+  double *tab = rows + NROW * ncols;
+  {
+    const int64_t ntab = rys_table_used(S::NR, cd.xswitch);
+    const double *src = rys_table + rys_offset[S::NR];
+    for (int64_t i = threadIdx.x; i < ntab; i += blockDim.x) tab[i] = src[i];
+  }
 
   const int tile = cd.t_begin + (int)blockIdx.x;
   const int64_t q0 = cd.tile_q[tile] > cd.q_begin ? cd.tile_q[tile] : cd.q_begin;
@@ -644,9 +665,9 @@ __global__ __launch_bounds__(256) void eri_row_kernel(ClassDev cd, BuildArgs ba)
     const PairRec &ket = TR ? fix : mov;
     double v[S::NABCD];
     if (mixed_low(ba, bra, ket))
-      small_integrals<float, LA, LB, LC, LD>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, v);
+      small_integrals<float, LA, LB, LC, LD>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, v, tab);
     else
-      small_integrals<double, LA, LB, LC, LD>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, v);
+      small_integrals<double, LA, LB, LC, LD>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, v, tab);
 
     double s = sfix;
     if (mov.sameAB) s *= 0.5;
@@ -1742,27 +1763,26 @@ int64_t row_acc_bytes(const int32_t *ncol) {
 // LDS bytes of the row-tiled form of the class in orientation TR (lane
 // classes only), or -1 when the class has none
 template <int LA, int LB, int LC, int LD, bool TR>
-int64_t row_lds(const int32_t *ncol) {
+int64_t row_lds(const int32_t *ncol, double xswitch) {
   using R = RowShape<LA, LB, LC, LD, TR>;
-  using GS = Grp<LA, LB, LC, LD>;
   if constexpr (!R::exists)
     return -1;
   else if constexpr (use_small<LA, LB, LC, LD>())
-    return row_acc_bytes<LA, LB, LC, LD, TR>(ncol);
+    return row_acc_bytes<LA, LB, LC, LD, TR>(ncol) +
+           (int64_t)sizeof(double) * rys_table_used(Small<LA, LB, LC, LD>::NR, xswitch);
   else
     return -1;
 }
 
 template <int LA, int LB, int LC, int LD, bool TR>
 static int launch_row(const ClassDev &cd, const BuildArgs &ba, hipStream_t st) {
-  const int64_t lds = row_lds<LA, LB, LC, LD, TR>(ba.ncol);
+  const int64_t lds = row_lds<LA, LB, LC, LD, TR>(ba.ncol, cd.xswitch);
   if (lds < 0 || lds > kRowLdsBytes || cd.t_end <= cd.t_begin) return hipErrorInvalidValue;
-  const unsigned acc = (unsigned)row_acc_bytes<LA, LB, LC, LD, TR>(ba.ncol);
   const dim3 grid((unsigned)(cd.t_end - cd.t_begin));
   if constexpr (!RowShape<LA, LB, LC, LD, TR>::exists) {
     return hipErrorInvalidValue;
   } else if constexpr (use_small<LA, LB, LC, LD>()) {
-    hipLaunchKernelGGL((eri_row_kernel<LA, LB, LC, LD, TR>), grid, dim3(256), acc, st, cd, ba);
+    hipLaunchKernelGGL((eri_row_kernel<LA, LB, LC, LD, TR>), grid, dim3(256), (unsigned)lds, st, cd, ba);
   } else {
     return hipErrorInvalidValue;
   }
@@ -1921,11 +1941,11 @@ int class_kernel_kind(int c) {
   }
 }
 
-int64_t class_row_lds(int c, int transposed, const int32_t *ncol) {
+int64_t class_row_lds(int c, int transposed, const int32_t *ncol, double xswitch) {
   switch (c) {
-#define X(id, a, b, cc, d)                                                   \
-  case id:                                                                   \
-    return transposed ? row_lds<a, b, cc, d, true>(ncol) : row_lds<a, b, cc, d, false>(ncol);
+#define X(id, a, b, cc, d)                                                                      \
+  case id:                                                                                      \
+    return transposed ? row_lds<a, b, cc, d, true>(ncol, xswitch) : row_lds<a, b, cc, d, false>(ncol, xswitch);
     LIBERI_CLASS_LIST(X)
 #undef X
     default:

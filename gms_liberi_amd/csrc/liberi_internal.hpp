@@ -120,8 +120,9 @@ int class_nroots(int c);
 double class_rys_xmax(int c);
 // LDS bytes of the row-tiled kernel of class c with rows over the bra pairs
 // (transposed = 0) or the ket pairs (1), given the compact column counts per
-// angular momentum; -1 when the class has no such kernel
-int64_t class_row_lds(int c, int transposed, const int32_t *ncol);
+// angular momentum and the Rys switch (the kernel stages the table below it);
+// -1 when the class has no such kernel
+int64_t class_row_lds(int c, int transposed, const int32_t *ncol, double xswitch);
 // kernel shape of class c: 0 lane per quartet, 1 workgroup group per
 // quartet, 2 cooperative workgroup per quartet
 int class_kernel_kind(int c);

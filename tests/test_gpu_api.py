@@ -189,8 +189,11 @@ def test_autotuned_build_matches_reference():
     active = v1 >= 0
     assert active.sum() > 20 and np.array_equal(v1, v2)
     assert np.array_equal(v3 >= 0, active)
+    # the forms differ only in the order of the FP64 additions into F: within
+    # 1e-13 of the largest element (elements of the chain reach ~1.3e3)
+    scale = max(1.0, float(np.max(np.abs(f1))))
     for f in (f1, f2, f3):
-        assert np.max(np.abs(f - f1)) <= 1e-11
+        assert np.max(np.abs(f - f1)) <= 1e-13 * scale
     dE = 0.5 * float(np.dot(case.density, f1 - case.fock))
     assert abs(dE) <= 1e-9
 
