@@ -1247,7 +1247,25 @@ struct Grp {
   static constexpr int SLOT = UNI + DB;
   static constexpr int NKC = (NE * NF + G - 1) / G;
   static constexpr bool fits = SLOT * NQ * 8 <= 64 * 1024;
+  // rule records of the primitive loop, one per primitive quartet:
+  // u[NR], w[NR] (times 2 pi^5/2 K_P K_Q / sqrt(p+q)), 1/(p+q), q/(p+q),
+  // p/(p+q), 1/2p, 1/2q.  They live in the part of the slot the tables leave
+  // free (HRR and density space, written only after the loop); RT of them per
+  // slot are made in one pass of the wave.
+  static constexpr int REC = 2 * NR + 5;
+  static constexpr int RT0 = (SLOT - 2 * GS) / REC;
+  static constexpr int RT = RT0 < 64 / NQ ? RT0 : 64 / NQ;
 };
+
+// The group kernels run one wave per workgroup.  A wave's LDS operations
+// complete in program order, so between the LDS phases of such a kernel a
+// compiler-level fence at wavefront scope is enough: no s_barrier and no
+// drain of the wave's outstanding memory operations.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 __device__ __forceinline__ int gpos_of(int L, int k) {
   int x, y, z;
@@ -1260,14 +1278,19 @@ __device__ __forceinline__ int gpos_of(int L, int k) {
 // per row tile in LDS rows, or per run of kets sharing a shell -- cut this
 // kernel's atomics 5-9x on C60 but not its time: its atomics overlap its
 // arithmetic, and the LDS rows cost occupancy.)
+// (classes up to total angular momentum 6 fit 168 VGPRs, i.e. 3 waves per
+// SIMD, without spilling; the hint keeps them there)
 template <int LA, int LB, int LC, int LD>
+__attribute__((amdgpu_waves_per_eu(LA + LB + LC + LD <= 6 ? 3 : 1, 8)))
 __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba) {
   using S = Grp<LA, LB, LC, LD>;
   constexpr int G = S::G, NQ = S::NQ, NR = S::NR, NI = S::NI, NK = S::NK, NE = S::NE, NF = S::NF, NKC = S::NKC;
   constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND, NABCD = S::NABCD;
   constexpr int GSZ = NI * NK;
+  constexpr int RT = S::RT, REC = S::REC;
+  static_assert(RT >= 1, "no room for the rule records");
   __shared__ double lds[NQ * S::SLOT];
-  __shared__ int s_npq[NQ];
+  __shared__ int s_npq[NQ], s_nk[NQ], s_boff[NQ], s_koff[NQ];
 
   const int lane = threadIdx.x, slot = lane / G, gl = lane - slot * G;
   double *Gt = lds + slot * S::SLOT;
@@ -1300,8 +1323,13 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
     }
     const int nk = valid ? ket.nprim : 0;
     const int npq = valid ? bra.nprim * ket.nprim : 0;
-    if (gl == 0) s_npq[slot] = npq;
-    __syncthreads();
+    if (gl == 0) {
+      s_npq[slot] = npq;
+      s_nk[slot] = nk;
+      s_boff[slot] = bra.prim_off;
+      s_koff[slot] = ket.prim_off;
+    }
+    wave_sync();
     int maxpq = 0;
 #pragma unroll
     for (int s = 0; s < NQ; ++s) maxpq = s_npq[s] > maxpq ? s_npq[s] : maxpq;
@@ -1310,45 +1338,81 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
 #pragma unroll
     for (int k = 0; k < NKC; ++k) acc[k] = 0.0;
 
-    // 2-D Rys tables of primitive quartet t (3 NR lanes: one root and one
-    // direction each) into dst
-    auto tables = [&](int t, double *dst) {
-      if (t < npq && gl < 3 * NR) {
-        const int ip = t / nk, iq = t - ip * nk;
-        const PrimRec P = ba.prims[bra.prim_off + ip];
-        const PrimRec Q = ba.prims[ket.prim_off + iq];
-        const int r = gl / 3, dim = gl - 3 * (gl / 3);
+    // rule records of primitive quartets t0 .. t0+RT-1 of every slot, one
+    // lane per (slot, quartet): the Rys rule, its prefactor and the
+    // reciprocals the recursion needs
+    auto rules = [&](int t0) {
+      const int sa = lane / RT, tt = lane - sa * RT, t = t0 + tt;
+      if (sa < NQ && t < s_npq[sa]) {
+        const int nka = s_nk[sa];
+        const int ip = t / nka, iq = t - ip * nka;
+        const PrimRec P = ba.prims[s_boff[sa] + ip];
+        const PrimRec Q = ba.prims[s_koff[sa] + iq];
         const double pq = P.p + Q.p, ipq = 1.0 / pq;
         const double pqx = P.Px - Q.Px, pqy = P.Py - Q.Py, pqz = P.Pz - Q.Pz;
         const double X = P.p * Q.p * ipq * (pqx * pqx + pqy * pqy + pqz * pqz);
-        double t2, wr;
-        rys_root<NR>(X, cd.xswitch, r, t2, wr);
-        const double hp = 0.5 / P.p, hq = 0.5 / Q.p;
-        const double b00 = 0.5 * t2 * ipq;
-        const double b10 = hp - hp * Q.p * t2 * ipq;
-        const double b01 = hq - hq * P.p * t2 * ipq;
-        const double fa = Q.p * ipq * t2, fc = P.p * ipq * t2;
-        const double pqd = dim == 0 ? pqx : (dim == 1 ? pqy : pqz);
-        const double pa = dim == 0 ? P.PAx : (dim == 1 ? P.PAy : P.PAz);
-        const double qc = dim == 0 ? Q.PAx : (dim == 1 ? Q.PAy : Q.PAz);
-        const double g00 = dim == 2 ? wr * kTwoPi52 * P.K * Q.K * sqrt(ipq) : 1.0;
+        const double pref = kTwoPi52 * P.K * Q.K * sqrt(ipq);
+        double *rec = lds + sa * S::SLOT + 2 * S::GS + tt * REC;
+#pragma unroll 1
+        for (int r = 0; r < NR; ++r) {
+          double ur, wr;
+          rys_root<NR>(X, cd.xswitch, r, ur, wr);
+          rec[r] = ur;
+          rec[NR + r] = wr * pref;
+        }
+        rec[2 * NR] = ipq;
+        rec[2 * NR + 1] = Q.p * ipq;
+        rec[2 * NR + 2] = P.p * ipq;
+        rec[2 * NR + 3] = 0.5 / P.p;
+        rec[2 * NR + 4] = 0.5 / Q.p;
+      }
+    };
+    // 2-D Rys tables of primitive quartet t (3 NR lanes: one root and one
+    // direction each) into dst, from its rule record; called for t = 0, 1, ...
+    // in order (ip, iq: the primitives of the next quartet)
+    int ip = 0, iq = 0;
+    auto tables = [&](int t, double *dst) {
+      if (t < npq && gl < 3 * NR) {
+        const int r = gl / 3, dim = gl - 3 * (gl / 3);
+        const double *rec = Gt + 2 * S::GS + (t % RT) * REC;
+        const double *pp = &ba.prims[bra.prim_off + ip].Px, *qq = &ba.prims[ket.prim_off + iq].Px;
+        const double u = rec[r], ipq = rec[2 * NR], a = rec[2 * NR + 1], c = rec[2 * NR + 2];
+        const double hp = rec[2 * NR + 3], hq = rec[2 * NR + 4];
+        const double fa = a * u, fc = c * u;
+        const double b00 = 0.5 * u * ipq, b10 = hp - hp * fa, b01 = hq - hq * fc;
+        const double pqd = pp[dim] - qq[dim];
+        const double g00 = dim == 2 ? rec[NR + r] : 1.0;
         double g[NI][NK];
-        rys_2d<NI, NK>(g, pa - fa * pqd, qc + fc * pqd, b10, b01, b00, g00);
+        rys_2d<NI, NK>(g, pp[3 + dim] - fa * pqd, qq[3 + dim] + fc * pqd, b10, b01, b00, g00);
 #pragma unroll
         for (int i = 0; i < NI; ++i)
 #pragma unroll
           for (int k = 0; k < NK; ++k) dst[(dim * NR + r) * GSZ + i * NK + k] = g[i][k];
       }
+      if (++iq == nk) {
+        iq = 0;
+        ++ip;
+      }
     };
     // software pipeline: while every lane accumulates quartet t from one
     // buffer, the table lanes fill the other with quartet t+1 -- one barrier
-    // per primitive quartet
-    if (maxpq > 0) tables(0, Gt);
-    __syncthreads();
+    // per primitive quartet (two when a new batch of rule records is due)
+    if (maxpq > 0) {
+      rules(0);
+      wave_sync();
+      tables(0, Gt);
+    }
+    wave_sync();
     for (int t = 0; t < maxpq; ++t) {
       const double *cur = (t & 1) ? Gt + S::GS : Gt;
       double *nxt = (t & 1) ? Gt : Gt + S::GS;
-      if (t + 1 < maxpq) tables(t + 1, nxt);
+      if (t + 1 < maxpq) {
+        if ((t + 1) % RT == 0) {  // the records of t+1 .. t+RT (the last batch's readers are past a barrier)
+          rules(t + 1);
+          wave_sync();
+        }
+        tables(t + 1, nxt);
+      }
       if (t < npq) {
 #pragma unroll
         for (int k = 0; k < NKC; ++k) {
@@ -1359,7 +1423,7 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
           acc[k] += v;
         }
       }
-      __syncthreads();
+      wave_sync();
     }
     // (e0|f0) to LDS: rows e of stride NF for the bra HRR, or directly the
     // ket-phase rows [a][KS] when there is no bra HRR
@@ -1372,7 +1436,7 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
         buf0[e * ACCS + (c - e * NF)] = acc[k];
       }
     }
-    __syncthreads();
+    wave_sync();
 
     // bra HRR, lanes over the columns f; the last level writes rows [ab] of
     // stride KS for the ket HRR, whose lanes run over the rows ab.  Component
@@ -1381,32 +1445,32 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
     double *in = buf0, *out = buf1;
     if constexpr (LB >= 1) {
       grp_bra_level<LA, S::LAB, 1, NF, G, (LB == 1 ? S::KS : NF)>(in, out, gl, bra.ABx, bra.ABy, bra.ABz);
-      __syncthreads();
+      wave_sync();
       double *t = in; in = out; out = t;
     }
     if constexpr (LB >= 2) {
       grp_bra_level<LA, S::LAB, 2, NF, G, (LB == 2 ? S::KS : NF)>(in, out, gl, bra.ABx, bra.ABy, bra.ABz);
-      __syncthreads();
+      wave_sync();
       double *t = in; in = out; out = t;
     }
     if constexpr (LB >= 3) {
       grp_bra_level<LA, S::LAB, 3, NF, G, S::KS>(in, out, gl, bra.ABx, bra.ABy, bra.ABz);
-      __syncthreads();
+      wave_sync();
       double *t = in; in = out; out = t;
     }
     if constexpr (LD >= 1) {
       grp_ket_level<LC, S::LCD, 1, S::NAB, G, S::KS>(in, out, gl, ket.ABx, ket.ABy, ket.ABz);
-      __syncthreads();
+      wave_sync();
       double *t = in; in = out; out = t;
     }
     if constexpr (LD >= 2) {
       grp_ket_level<LC, S::LCD, 2, S::NAB, G, S::KS>(in, out, gl, ket.ABx, ket.ABy, ket.ABz);
-      __syncthreads();
+      wave_sync();
       double *t = in; in = out; out = t;
     }
     if constexpr (LD >= 3) {
       grp_ket_level<LC, S::LCD, 3, S::NAB, G, S::KS>(in, out, gl, ket.ABx, ket.ABy, ket.ABz);
-      __syncthreads();
+      wave_sync();
       double *t = in; in = out; out = t;
     }
     constexpr int KS = S::KS, NCD = NC * ND;
@@ -1434,7 +1498,7 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
       for (int i = gl; i < S::NAD; i += G) DAD[i] = Dk[tri(bra.bfA + gpos_rt(LA, i / ND), ket.bfB + gpos_rt(LD, i % ND))];
       for (int i = gl; i < S::NBC; i += G) DBC[i] = Dk[tri(bra.bfB + gpos_rt(LB, i / NC), ket.bfA + gpos_rt(LC, i % NC))];
     }
-    __syncthreads();
+    wave_sync();
     if (valid) {
       double s = 1.0;
       if (bra.sameAB) s *= 0.5;
@@ -1507,7 +1571,7 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
         put(bra.bfB + pb, ket.bfA + pc, -s * ba.k_scale * t);
       }
     }
-    __syncthreads();  // slots are reused by the next round
+    wave_sync();  // slots are reused by the next round
   };
 
   const int64_t nq = cd.q_end - cd.q_begin;
