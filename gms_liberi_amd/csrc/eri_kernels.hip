@@ -1222,10 +1222,25 @@ __device__ __forceinline__ void grp_ket_level(const double *__restrict__ in, dou
   }
 }
 
+// LDS bytes of a one-wave group workgroup that still allow 3 of them per SIMD
+constexpr int kGroupLdsTarget = 160 * 1024 / 12;
+
+// Root split P of the row/column lane map of the primitive loop (lanes per
+// quartet = nl * P): the smallest split whose NQ = 64 / (nl P) slots of
+// `slot` doubles fit kGroupLdsTarget, else the largest split that fits a
+// wave; the 3 NR table lanes must fit the group.  0 when no split fits.
+__host__ __device__ constexpr int grp_parts(int nl, int nr, int slot) {
+  int best = 0;
+  for (int p = 1; p <= nr && nl * p <= 64; ++p) {
+    if (nl * p < 3 * nr) continue;
+    best = p;
+    if ((64 / (nl * p)) * slot * 8 <= kGroupLdsTarget) return p;
+  }
+  return best;
+}
+
 template <int LA, int LB, int LC, int LD>
 struct Grp {
-  static constexpr int G = GroupSize<LA, LB, LC, LD>::G;
-  static constexpr int NQ = 64 / G;
   static constexpr int LAB = LA + LB, LCD = LC + LD;
   static constexpr int NR = (LAB + LCD) / 2 + 1;
   static constexpr int NI = LAB + 1, NK = LCD + 1;
@@ -1245,6 +1260,22 @@ struct Grp {
   // ends: the HRR buffers alias them
   static constexpr int UNI = 2 * GS > 2 * BUFP ? 2 * GS : 2 * BUFP;
   static constexpr int SLOT = UNI + DB;
+  // lane map of the primitive loop.  MAP 1: a lane per row e of (e0|f0)
+  // accumulating all its columns f; MAP 2: a lane per column f accumulating
+  // all rows e; in both the roots are split over P lanes (partial sums added
+  // at the end), and a lane reads one row (column) of each 2-D table per root
+  // at compile-time offsets.  MAP 0 (rows or columns longer than a wave): the
+  // NE NF components strided over G lanes, three table reads per product.
+  // The map is taken only where it keeps at least as many quartets per wave
+  // as the strided one (the table lanes' work per primitive quartet is the
+  // same for any map, so fewer quartets per wave cost more than the saved
+  // reads: e.g. (ps|dd), rows of 3).
+  static constexpr int NL = NE >= NF ? NE : NF;
+  static constexpr int P0 = NL <= 64 ? grp_parts(NL, NR, SLOT) : 0;
+  static constexpr int P = P0 && 64 / (NL * P0) >= 64 / GroupSize<LA, LB, LC, LD>::G ? P0 : 0;
+  static constexpr int MAP = P == 0 ? 0 : (NE >= NF ? 1 : 2);
+  static constexpr int G = MAP ? NL * P : GroupSize<LA, LB, LC, LD>::G;
+  static constexpr int NQ = 64 / G;
   static constexpr int NKC = (NE * NF + G - 1) / G;
   static constexpr bool fits = SLOT * NQ * 8 <= 64 * 1024;
   // rule records of the primitive loop, one per primitive quartet:
@@ -1278,10 +1309,10 @@ __device__ __forceinline__ int gpos_of(int L, int k) {
 // per row tile in LDS rows, or per run of kets sharing a shell -- cut this
 // kernel's atomics 5-9x on C60 but not its time: its atomics overlap its
 // arithmetic, and the LDS rows cost occupancy.)
-// (classes up to total angular momentum 6 fit 168 VGPRs, i.e. 3 waves per
-// SIMD, without spilling; the hint keeps them there)
+// (classes of s, p and d shells up to total angular momentum 6 fit 168
+// VGPRs, i.e. 3 waves per SIMD, without spilling; the hint keeps them there)
 template <int LA, int LB, int LC, int LD>
-__attribute__((amdgpu_waves_per_eu(LA + LB + LC + LD <= 6 ? 3 : 1, 8)))
+__attribute__((amdgpu_waves_per_eu(LA <= 2 && LB <= 2 && LC <= 2 && LD <= 2 && LA + LB + LC + LD <= 6 ? 3 : 1, 8)))
 __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba) {
   using S = Grp<LA, LB, LC, LD>;
   constexpr int G = S::G, NQ = S::NQ, NR = S::NR, NI = S::NI, NK = S::NK, NE = S::NE, NF = S::NF, NKC = S::NKC;
@@ -1292,17 +1323,23 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
   __shared__ double lds[NQ * S::SLOT];
   __shared__ int s_npq[NQ], s_nk[NQ], s_boff[NQ], s_koff[NQ];
 
-  const int lane = threadIdx.x, slot = lane / G, gl = lane - slot * G;
+  // lanes past the NQ groups (G need not divide 64) idle: their gl lies past
+  // every loop bound and their slots are never valid
+  const int lane = threadIdx.x;
+  const bool idle = lane >= NQ * G;
+  const int slot = idle ? NQ - 1 : lane / G;
+  const int gl = idle ? (1 << 20) : lane - slot * G;
   double *Gt = lds + slot * S::SLOT;
   double *buf0 = Gt, *buf1 = buf0 + S::BUFP, *Db = Gt + S::UNI;
   double *DAB = Db, *DCD = DAB + S::NAB, *DAC = DCD + S::NCD, *DBD = DAC + S::NAC, *DAD = DBD + S::NBD,
          *DBC = DAD + S::NAD;
 
+  // MAP 0: table offsets of the lane's components
   int offx[NKC], offy[NKC], offz[NKC];
 #pragma unroll
   for (int k = 0; k < NKC; ++k) {
     const int c = gl + G * k;
-    if (c < NE * NF) {
+    if (S::MAP == 0 && c < NE * NF) {
       int Le, ex, ey, ez, Lf, fx, fy, fz;
       range_decode(LA, c / NF, Le, ex, ey, ez);
       range_decode(LC, c % NF, Lf, fx, fy, fz);
@@ -1312,7 +1349,17 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
     } else {
       offx[k] = offy[k] = offz[k] = 0;
     }
+  }  // MAP 1/2: the lane's row e (column f) and root part
+  constexpr int P = S::P, NRP = P ? (NR + P - 1) / P : 1;
+  const int part = P ? gl % P : 0;
+  int rx = 0, ry = 0, rz = 0;
+  if (S::MAP != 0 && gl < G) {
+    int Lr;
+    range_decode(S::MAP == 1 ? LA : LC, gl / P, Lr, rx, ry, rz);
   }
+  // table element of the lane's row (MAP 1: [rx][j]) or column (MAP 2: [j][rx])
+  const int rowx = S::MAP == 1 ? rx * NK : rx, rowy = S::MAP == 1 ? ry * NK : ry, rowz = S::MAP == 1 ? rz * NK : rz;
+
 
   // one round of NQ quartets (one per slot): integrals, HRR, digestion
   auto round = [&](bool valid, int32_t ib, int32_t ik) {
@@ -1334,9 +1381,10 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
 #pragma unroll
     for (int s = 0; s < NQ; ++s) maxpq = s_npq[s] > maxpq ? s_npq[s] : maxpq;
 
-    double acc[NKC];
+    constexpr int NACC = S::MAP == 0 ? NKC : (S::MAP == 1 ? NF : NE);
+    double acc[NACC];
 #pragma unroll
-    for (int k = 0; k < NKC; ++k) acc[k] = 0.0;
+    for (int k = 0; k < NACC; ++k) acc[k] = 0.0;
 
     // rule records of primitive quartets t0 .. t0+RT-1 of every slot, one
     // lane per (slot, quartet): the Rys rule, its prefactor and the
@@ -1413,14 +1461,39 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
         }
         tables(t + 1, nxt);
       }
-      if (t < npq) {
+      if constexpr (S::MAP == 0) {
+        if (t < npq) {
 #pragma unroll
-        for (int k = 0; k < NKC; ++k) {
-          double v = 0.0;
+          for (int k = 0; k < NKC; ++k) {
+            double v = 0.0;
 #pragma unroll
-          for (int r = 0; r < NR; ++r)
-            v += cur[r * GSZ + offx[k]] * cur[(NR + r) * GSZ + offy[k]] * cur[(2 * NR + r) * GSZ + offz[k]];
-          acc[k] += v;
+            for (int r = 0; r < NR; ++r)
+              v += cur[r * GSZ + offx[k]] * cur[(NR + r) * GSZ + offy[k]] * cur[(2 * NR + r) * GSZ + offz[k]];
+            acc[k] += v;
+          }
+        }
+      } else {
+        if (t < npq && gl < G) {
+          // MAP 1: loop over the columns f; MAP 2: over the rows e
+          constexpr int L0 = S::MAP == 1 ? LC : LA, L1 = S::MAP == 1 ? S::LCD : S::LAB;
+          constexpr int NJ = S::MAP == 1 ? NK : NI, SJ = S::MAP == 1 ? 1 : NK;
+          constexpr Range<L0, L1> rl{};
+          const double *bx = cur + part * GSZ + rowx, *by = cur + (NR + part) * GSZ + rowy,
+                       *bz = cur + (2 * NR + part) * GSZ + rowz;
+#pragma unroll
+          for (int rr = 0; rr < NRP; ++rr) {
+            if (NR % P == 0 || part + rr * P < NR) {
+              double X[NJ], Y[NJ], Z[NJ];
+#pragma unroll
+              for (int j = 0; j < NJ; ++j) {
+                X[j] = bx[rr * P * GSZ + j * SJ];
+                Y[j] = by[rr * P * GSZ + j * SJ];
+                Z[j] = bz[rr * P * GSZ + j * SJ];
+              }
+#pragma unroll
+              for (int k = 0; k < NACC; ++k) acc[k] += X[rl.lx[k]] * Y[rl.ly[k]] * Z[rl.lz[k]];
+            }
+          }
         }
       }
       wave_sync();
@@ -1428,12 +1501,33 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
     // (e0|f0) to LDS: rows e of stride NF for the bra HRR, or directly the
     // ket-phase rows [a][KS] when there is no bra HRR
     constexpr int ACCS = LB > 0 ? NF : S::KS;
+    if constexpr (S::MAP == 0) {
 #pragma unroll
-    for (int k = 0; k < NKC; ++k) {
-      const int c = gl + G * k;
-      if (c < NE * NF) {
-        const int e = c / NF;
-        buf0[e * ACCS + (c - e * NF)] = acc[k];
+      for (int k = 0; k < NKC; ++k) {
+        const int c = gl + G * k;
+        if (c < NE * NF) {
+          const int e = c / NF;
+          buf0[e * ACCS + (c - e * NF)] = acc[k];
+        }
+      }
+    } else {
+      // the P partial sums of a row (column) sit on adjacent lanes
+#pragma unroll
+      for (int k = 0; k < NACC; ++k) {
+        double v = acc[k];
+#pragma unroll
+        for (int o = 1; o < P; ++o) v += __shfl(acc[k], lane + o, 64);
+        acc[k] = v;
+      }
+      if (gl < G && part == 0) {
+        const int rl = gl / P;
+#pragma unroll
+        for (int k = 0; k < NACC; ++k) {
+          if (S::MAP == 1)
+            buf0[rl * ACCS + k] = acc[k];
+          else
+            buf0[k * ACCS + rl] = acc[k];
+        }
       }
     }
     wave_sync();
@@ -1576,7 +1670,7 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
 
   const int64_t nq = cd.q_end - cd.q_begin;
   for (int64_t base = (int64_t)blockIdx.x * NQ; base < nq; base += (int64_t)gridDim.x * NQ) {
-    const bool valid = base + slot < nq;
+    const bool valid = !idle && base + slot < nq;
     int32_t ib = 0, ik = 0;
     if (valid) decode_quartet(cd, cd.q_begin + base + slot, ib, ik);
     round(valid, ib, ik);
