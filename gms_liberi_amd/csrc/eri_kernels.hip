@@ -1278,14 +1278,20 @@ struct Grp {
   static constexpr int NQ = 64 / G;
   static constexpr int NKC = (NE * NF + G - 1) / G;
   static constexpr bool fits = SLOT * NQ * 8 <= 64 * 1024;
+  // primitive quartets whose 2-D tables one pass of the table lanes makes
+  // (3 NR lanes each), held in the HRR space of the slot
+  static constexpr int KT0 = G / (3 * NR) < UNI / GS ? G / (3 * NR) : UNI / GS;
+  static constexpr int KT = KT0 < 1 ? 1 : KT0;
   // rule records of the primitive loop, one per primitive quartet:
   // u[NR], w[NR] (times 2 pi^5/2 K_P K_Q / sqrt(p+q)), 1/(p+q), q/(p+q),
-  // p/(p+q), 1/2p, 1/2q.  They live in the part of the slot the tables leave
-  // free (HRR and density space, written only after the loop); RT of them per
-  // slot are made in one pass of the wave.
-  static constexpr int REC = 2 * NR + 5;
-  static constexpr int RT0 = (SLOT - 2 * GS) / REC;
-  static constexpr int RT = RT0 < 64 / NQ ? RT0 : 64 / NQ;
+  // p/(p+q), 1/2p, 1/2q, P-Q, P-A, Q-C.  They live in the part of the slot
+  // the tables leave free (HRR and density space, written only after the
+  // loop); RT of them per slot (a multiple of KT) are made in one pass of the
+  // wave.
+  static constexpr int REC = 2 * NR + 14;
+  static constexpr int RT0 = (SLOT - KT * GS) / REC;
+  static constexpr int RT1 = RT0 < 64 / NQ ? RT0 : 64 / NQ;
+  static constexpr int RT = RT1 / KT * KT;
 };
 
 // The group kernels run one wave per workgroup.  A wave's LDS operations
@@ -1318,8 +1324,8 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
   constexpr int G = S::G, NQ = S::NQ, NR = S::NR, NI = S::NI, NK = S::NK, NE = S::NE, NF = S::NF, NKC = S::NKC;
   constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND, NABCD = S::NABCD;
   constexpr int GSZ = NI * NK;
-  constexpr int RT = S::RT, REC = S::REC;
-  static_assert(RT >= 1, "no room for the rule records");
+  constexpr int RT = S::RT, REC = S::REC, KT = S::KT;
+  static_assert(RT >= KT && KT * S::GS <= S::UNI, "no room for the tables and rule records");
   __shared__ double lds[NQ * S::SLOT];
   __shared__ int s_npq[NQ], s_nk[NQ], s_boff[NQ], s_koff[NQ];
 
@@ -1387,8 +1393,8 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
     for (int k = 0; k < NACC; ++k) acc[k] = 0.0;
 
     // rule records of primitive quartets t0 .. t0+RT-1 of every slot, one
-    // lane per (slot, quartet): the Rys rule, its prefactor and the
-    // reciprocals the recursion needs
+    // lane per (slot, quartet): the Rys rule, its prefactor, the reciprocals
+    // and the displacements the recursion needs
     auto rules = [&](int t0) {
       const int sa = lane / RT, tt = lane - sa * RT, t = t0 + tt;
       if (sa < NQ && t < s_npq[sa]) {
@@ -1400,7 +1406,7 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
         const double pqx = P.Px - Q.Px, pqy = P.Py - Q.Py, pqz = P.Pz - Q.Pz;
         const double X = P.p * Q.p * ipq * (pqx * pqx + pqy * pqy + pqz * pqz);
         const double pref = kTwoPi52 * P.K * Q.K * sqrt(ipq);
-        double *rec = lds + sa * S::SLOT + 2 * S::GS + tt * REC;
+        double *rec = lds + sa * S::SLOT + KT * S::GS + tt * REC;
 #pragma unroll 1
         for (int r = 0; r < NR; ++r) {
           double ur, wr;
@@ -1413,85 +1419,85 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
         rec[2 * NR + 2] = P.p * ipq;
         rec[2 * NR + 3] = 0.5 / P.p;
         rec[2 * NR + 4] = 0.5 / Q.p;
+        rec[2 * NR + 5] = pqx;
+        rec[2 * NR + 6] = pqy;
+        rec[2 * NR + 7] = pqz;
+        rec[2 * NR + 8] = P.PAx;
+        rec[2 * NR + 9] = P.PAy;
+        rec[2 * NR + 10] = P.PAz;
+        rec[2 * NR + 11] = Q.PAx;
+        rec[2 * NR + 12] = Q.PAy;
+        rec[2 * NR + 13] = Q.PAz;
       }
     };
-    // 2-D Rys tables of primitive quartet t (3 NR lanes: one root and one
-    // direction each) into dst, from its rule record; called for t = 0, 1, ...
-    // in order (ip, iq: the primitives of the next quartet)
-    int ip = 0, iq = 0;
-    auto tables = [&](int t, double *dst) {
-      if (t < npq && gl < 3 * NR) {
-        const int r = gl / 3, dim = gl - 3 * (gl / 3);
-        const double *rec = Gt + 2 * S::GS + (t % RT) * REC;
-        const double *pp = &ba.prims[bra.prim_off + ip].Px, *qq = &ba.prims[ket.prim_off + iq].Px;
+    // 2-D Rys tables of primitive quartets t0 .. t0+KT-1 (3 NR lanes each:
+    // one root and one direction per lane) from their rule records, quartet
+    // t0+tt into table buffer tt
+    auto tables = [&](int t0) {
+      const int tt = gl / (3 * NR), task = gl - tt * (3 * NR), t = t0 + tt;
+      if (tt < KT && t < npq) {
+        const int r = task / 3, dim = task - 3 * r;
+        const double *rec = Gt + KT * S::GS + (t % RT) * REC;
         const double u = rec[r], ipq = rec[2 * NR], a = rec[2 * NR + 1], c = rec[2 * NR + 2];
         const double hp = rec[2 * NR + 3], hq = rec[2 * NR + 4];
+        const double pqd = rec[2 * NR + 5 + dim], pa = rec[2 * NR + 8 + dim], qc = rec[2 * NR + 11 + dim];
         const double fa = a * u, fc = c * u;
         const double b00 = 0.5 * u * ipq, b10 = hp - hp * fa, b01 = hq - hq * fc;
-        const double pqd = pp[dim] - qq[dim];
         const double g00 = dim == 2 ? rec[NR + r] : 1.0;
         double g[NI][NK];
-        rys_2d<NI, NK>(g, pp[3 + dim] - fa * pqd, qq[3 + dim] + fc * pqd, b10, b01, b00, g00);
+        rys_2d<NI, NK>(g, pa - fa * pqd, qc + fc * pqd, b10, b01, b00, g00);
+        double *dst = Gt + tt * S::GS + (dim * NR + r) * GSZ;
 #pragma unroll
         for (int i = 0; i < NI; ++i)
 #pragma unroll
-          for (int k = 0; k < NK; ++k) dst[(dim * NR + r) * GSZ + i * NK + k] = g[i][k];
-      }
-      if (++iq == nk) {
-        iq = 0;
-        ++ip;
+          for (int k = 0; k < NK; ++k) dst[i * NK + k] = g[i][k];
       }
     };
-    // software pipeline: while every lane accumulates quartet t from one
-    // buffer, the table lanes fill the other with quartet t+1 -- one barrier
-    // per primitive quartet (two when a new batch of rule records is due)
-    if (maxpq > 0) {
-      rules(0);
-      wave_sync();
-      tables(0, Gt);
-    }
-    wave_sync();
-    for (int t = 0; t < maxpq; ++t) {
-      const double *cur = (t & 1) ? Gt + S::GS : Gt;
-      double *nxt = (t & 1) ? Gt : Gt + S::GS;
-      if (t + 1 < maxpq) {
-        if ((t + 1) % RT == 0) {  // the records of t+1 .. t+RT (the last batch's readers are past a barrier)
-          rules(t + 1);
-          wave_sync();
-        }
-        tables(t + 1, nxt);
+    // KT primitive quartets per pass: (rule records when a batch is due),
+    // tables, then every lane accumulates them
+    for (int t0 = 0; t0 < maxpq; t0 += KT) {
+      if (t0 % RT == 0) {  // the last batch's readers are past a fence
+        rules(t0);
+        wave_sync();
       }
-      if constexpr (S::MAP == 0) {
-        if (t < npq) {
+      tables(t0);
+      wave_sync();
 #pragma unroll
-          for (int k = 0; k < NKC; ++k) {
-            double v = 0.0;
+      for (int tt = 0; tt < KT; ++tt) {
+        const int t = t0 + tt;
+        const double *cur = Gt + tt * S::GS;
+        if constexpr (S::MAP == 0) {
+          if (t < npq) {
 #pragma unroll
-            for (int r = 0; r < NR; ++r)
-              v += cur[r * GSZ + offx[k]] * cur[(NR + r) * GSZ + offy[k]] * cur[(2 * NR + r) * GSZ + offz[k]];
-            acc[k] += v;
+            for (int k = 0; k < NKC; ++k) {
+              double v = 0.0;
+#pragma unroll
+              for (int r = 0; r < NR; ++r)
+                v += cur[r * GSZ + offx[k]] * cur[(NR + r) * GSZ + offy[k]] * cur[(2 * NR + r) * GSZ + offz[k]];
+              acc[k] += v;
+            }
           }
-        }
-      } else {
-        if (t < npq && gl < G) {
-          // MAP 1: loop over the columns f; MAP 2: over the rows e
-          constexpr int L0 = S::MAP == 1 ? LC : LA, L1 = S::MAP == 1 ? S::LCD : S::LAB;
-          constexpr int NJ = S::MAP == 1 ? NK : NI, SJ = S::MAP == 1 ? 1 : NK;
-          constexpr Range<L0, L1> rl{};
-          const double *bx = cur + part * GSZ + rowx, *by = cur + (NR + part) * GSZ + rowy,
-                       *bz = cur + (2 * NR + part) * GSZ + rowz;
+        } else {
+          if (t < npq && gl < G) {
+            // MAP 1: loop over the columns f; MAP 2: over the rows e
+            constexpr int L0 = S::MAP == 1 ? LC : LA, L1 = S::MAP == 1 ? S::LCD : S::LAB;
+            constexpr int NJ = S::MAP == 1 ? NK : NI, SJ = S::MAP == 1 ? 1 : NK;
+            constexpr Range<L0, L1> rl{};
+            const double *bx = cur + part * GSZ + rowx, *by = cur + (NR + part) * GSZ + rowy,
+                         *bz = cur + (2 * NR + part) * GSZ + rowz;
 #pragma unroll
-          for (int rr = 0; rr < NRP; ++rr) {
-            if (NR % P == 0 || part + rr * P < NR) {
-              double X[NJ], Y[NJ], Z[NJ];
+            for (int rr = 0; rr < NRP; ++rr) {
+              if (NR % P == 0 || part + rr * P < NR) {
+                double X[NJ], Y[NJ], Z[NJ];
 #pragma unroll
-              for (int j = 0; j < NJ; ++j) {
-                X[j] = bx[rr * P * GSZ + j * SJ];
-                Y[j] = by[rr * P * GSZ + j * SJ];
-                Z[j] = bz[rr * P * GSZ + j * SJ];
+                for (int j = 0; j < NJ; ++j) {
+                  X[j] = bx[rr * P * GSZ + j * SJ];
+                  Y[j] = by[rr * P * GSZ + j * SJ];
+                  Z[j] = bz[rr * P * GSZ + j * SJ];
+                }
+#pragma unroll
+                for (int k = 0; k < NACC; ++k) acc[k] += X[rl.lx[k]] * Y[rl.ly[k]] * Z[rl.lz[k]];
               }
-#pragma unroll
-              for (int k = 0; k < NACC; ++k) acc[k] += X[rl.lx[k]] * Y[rl.ly[k]] * Z[rl.lz[k]];
             }
           }
         }
