@@ -1674,11 +1674,23 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
     wave_sync();  // slots are reused by the next round
   };
 
+  // a contiguous chunk of the class's quartets per workgroup (a multiple of
+  // NQ): consecutive rounds stay in the same or the next segment, so each
+  // lane searches the segments once and then steps forward, and they share
+  // the bra pair (its records and density blocks stay in cache)
   const int64_t nq = cd.q_end - cd.q_begin;
-  for (int64_t base = (int64_t)blockIdx.x * NQ; base < nq; base += (int64_t)gridDim.x * NQ) {
-    const bool valid = !idle && base + slot < nq;
+  const int64_t per = ((nq + gridDim.x - 1) / gridDim.x + NQ - 1) / NQ * NQ;
+  const int64_t c0 = (int64_t)blockIdx.x * per, c1 = c0 + per < nq ? c0 + per : nq;
+  int seg = -1;
+  for (int64_t base = c0; base < c1; base += NQ) {
+    const bool valid = !idle && base + slot < c1;
     int32_t ib = 0, ik = 0;
-    if (valid) decode_quartet(cd, cd.q_begin + base + slot, ib, ik);
+    if (valid) {
+      const int64_t q = cd.q_begin + base + slot;
+      if (seg < 0) seg = find_segment(cd, q, 0, cd.nseg);
+      while (seg + 1 < cd.nseg && cd.seg_start[seg + 1] <= q) ++seg;
+      quartet_of_segment(cd, seg, q, ib, ik);
+    }
     round(valid, ib, ik);
   }
 }
