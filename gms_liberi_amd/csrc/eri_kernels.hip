@@ -97,6 +97,26 @@ __device__ __forceinline__ int64_t tri(int64_t i, int64_t j) {
   return i >= j ? ((i * (i + 1)) >> 1) + j : ((j * (j + 1)) >> 1) + i;
 }
 
+// Reciprocal and reciprocal square root for the primitive loops: the
+// hardware estimate (v_rcp_f64 / v_rsq_f64) refined by two Newton-Raphson
+// steps, r <- r + r (1 - x r) and r <- r + r/2 (1 - x r^2), which leave the
+// result within a few ulp of the correctly rounded value (the IEEE divide
+// and square root sequences cost about twice as many instructions).
+__device__ __forceinline__ double nr_rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.0), r, r);
+  return fma(fma(-x, r, 1.0), r, r);
+}
+__device__ __forceinline__ double nr_rsqrt(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+  r = fma(0.5 * r, fma(-x * r, r, 1.0), r);
+  return fma(0.5 * r, fma(-x * r, r, 1.0), r);
+}
+__device__ __forceinline__ float nr_rcp(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  return fmaf(fmaf(-x, r, 1.0f), r, r);
+}
+
 // --------------------------------------------------------------- Rys rule
 // Tabulated rule for X < xswitch, asymptotic (Hermite) rule above.  xswitch
 // never exceeds rys_xmax[N]; the asymptotic rule reproduces exactly the
@@ -346,15 +366,15 @@ __device__ __forceinline__ void small_integrals(const PairRec &bra, const PairRe
     const PrimRec P = prims[bra.prim_off + ip];
     for (int iq = 0; iq < ket.nprim; ++iq) {
       const PrimRec Q = prims[ket.prim_off + iq];
-      const double pq = P.p + Q.p, ipq = 1.0 / pq;
+      const double rs = nr_rsqrt(P.p + Q.p), ipq = rs * rs;  // 1/sqrt(p+q), 1/(p+q)
       const double pqx = P.Px - Q.Px, pqy = P.Py - Q.Py, pqz = P.Pz - Q.Pz;
       const double rho = P.p * Q.p * ipq;
       const double X = rho * (pqx * pqx + pqy * pqy + pqz * pqz);
-      const double pref = kTwoPi52 * P.K * Q.K * sqrt(ipq);
+      const double pref = kTwoPi52 * P.K * Q.K * rs;
       double u[NR], w[NR];
       rys_rule<NR>(X, xswitch, u, w, tab);
       const T tp = (T)P.p, tq = (T)Q.p, tipq = (T)ipq;
-      const T hp = (T)0.5 / tp, hq = (T)0.5 / tq;
+      const T hp = (T)0.5 * nr_rcp(tp), hq = (T)0.5 * nr_rcp(tq);
       const T tpqx = (T)pqx, tpqy = (T)pqy, tpqz = (T)pqz;
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
@@ -1367,6 +1387,23 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
   const int rowx = S::MAP == 1 ? rx * NK : rx, rowy = S::MAP == 1 ? ry * NK : ry, rowz = S::MAP == 1 ? rz * NK : rz;
 
 
+  // the slot's J(AB) block for the run of quartets sharing bra pair jab_pair
+  constexpr int NJL = (S::NAB + G - 1) / G;
+  double jab[NJL];
+#pragma unroll
+  for (int k = 0; k < NJL; ++k) jab[k] = 0.0;
+  int32_t jab_pair = -1, jab_a = 0, jab_b = 0;
+  auto flush_jab = [&]() {
+    if (jab_pair < 0) return;
+#pragma unroll
+    for (int k = 0; k < NJL; ++k) {
+      const int i = gl + k * G;
+      if (i < S::NAB && jab[k] != 0.0)
+        fock_add(ba.F, jab_a + gpos_rt(LA, i / S::NB), jab_b + gpos_rt(LB, i % S::NB), jab[k]);
+      jab[k] = 0.0;
+    }
+  };
+
   // one round of NQ quartets (one per slot): integrals, HRR, digestion
   auto round = [&](bool valid, int32_t ib, int32_t ik) {
     PairRec bra{}, ket{};
@@ -1606,17 +1643,26 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
       if (cd.diag && ib == ik) s *= 0.5;
       const double *__restrict__ v = in;
       double *F = ba.F;
-      // adds x to F[i,j], or to its on-chip accumulator in the row modes
       auto put = [&](int64_t fi, int64_t fj, double x) {
         if (x != 0.0) fock_add(F, fi, fj, x);
       };
-      // J(AB): F[ab] += 4s sum_cd v D[cd]
-      for (int i = gl; i < S::NAB; i += G) {
-        const int a = i / NB, b = i - a * NB;
-        double t = 0.0;
+      // J(AB): F[ab] += 4s sum_cd v D[cd], summed in registers while the
+      // slot's bra pair stays the same
+      if (jab_pair != ib) {
+        flush_jab();
+        jab_pair = ib;
+        jab_a = bra.bfA;
+        jab_b = bra.bfB;
+      }
 #pragma unroll
-        for (int e = 0; e < NCD; ++e) t += v[i * KS + e] * DCD[e];
-        put(bra.bfA + gpos_rt(LA, a), bra.bfB + gpos_rt(LB, b), 4.0 * s * t);
+      for (int k = 0; k < NJL; ++k) {
+        const int i = gl + k * G;
+        if (i < S::NAB) {
+          double t = 0.0;
+#pragma unroll
+          for (int e = 0; e < NCD; ++e) t += v[i * KS + e] * DCD[e];
+          jab[k] += 4.0 * s * t;
+        }
       }
       // J(CD): F[cd] += 4s sum_ab v D[ab]
       for (int i = gl; i < NCD; i += G) {
@@ -1693,6 +1739,7 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
     }
     round(valid, ib, ik);
   }
+  flush_jab();
 }
 
 // ----------------------------------------------------- segment lengths
