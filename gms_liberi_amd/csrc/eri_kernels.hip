@@ -328,16 +328,22 @@ __device__ __forceinline__ void hrr_ket(T *buf0, T *buf1, T cdx, T cdy, T cdz) {
   if constexpr (LD >= 3) hrr_ket_level<LC, LC + LD, 3, NAB>(buf0, buf1, cdx, cdy, cdz);
 }
 
-// Mixed precision: a quartet whose Schwarz bound times max|D| is below tau is
-// evaluated in FP32 (tau = 0: never).  The count of such quartets is kept per
-// wave (one atomic add per wave).
-__device__ __forceinline__ bool mixed_low(const BuildArgs &ba, const PairRec &bra, const PairRec &ket) {
+// Mixed precision: quartets whose Schwarz bound times max|D| is below tau
+// may be evaluated in FP32 (tau = 0: never).  The form is chosen per wave --
+// FP32 only when the quartet of every active lane qualifies -- so that a wave
+// never runs both forms of the primitive loop.
+__device__ __forceinline__ bool mixed_wave(const BuildArgs &ba, const PairRec &bra, const PairRec &ket) {
   if (!(ba.fp32_tau > 0.0)) return false;
   const bool lo = bra.x * ket.x * ba.dmax[0] < ba.fp32_tau;
-  const uint64_t m = __ballot(lo);
-  if (m != 0 && __lane_id() == __ffsll((unsigned long long)m) - 1)
-    atomicAdd((unsigned long long *)ba.fp32_count, (unsigned long long)__popcll(m));
-  return lo;
+  return __ballot(!lo) == 0;
+}
+// adds the wave's FP32 quartets (n per lane) to the build's count: one atomic
+// per wave; every lane of the wave must call it
+__device__ __forceinline__ void count_fp32(const BuildArgs &ba, unsigned n) {
+  if (!(ba.fp32_tau > 0.0)) return;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
+  if (__lane_id() == 0 && n != 0) atomicAdd((unsigned long long *)ba.fp32_count, (unsigned long long)n);
 }
 
 // One contracted quartet held by one lane: primitive loop with Rys quadrature,
@@ -455,10 +461,14 @@ __global__ __launch_bounds__(256) void eri_small_kernel(ClassDev cd, BuildArgs b
   const PairRec bra = ba.pairs[ib];
   const PairRec ket = ba.pairs[ik];
   double res[S::NABCD];
-  if (mixed_low(ba, bra, ket))
+  if (mixed_wave(ba, bra, ket)) {
+    const uint64_t m = __ballot(true);
+    if (__lane_id() == __ffsll((unsigned long long)m) - 1)
+      atomicAdd((unsigned long long *)ba.fp32_count, (unsigned long long)__popcll(m));
     small_integrals<float, LA, LB, LC, LD>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, res);
-  else
+  } else {
     small_integrals<double, LA, LB, LC, LD>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, res);
+  }
 
   // degeneracy weight of the shell quartet (A==B, C==D, AB==CD)
   double s = 1.0;
@@ -674,6 +684,7 @@ __global__ __launch_bounds__(256) void eri_row_kernel(ClassDev cd, BuildArgs ba)
   const double sfix = fix.sameAB ? 0.5 : 1.0;
   __syncthreads();
 
+  unsigned nlow = 0;  // quartets this lane evaluated in FP32
   for (int64_t qb = q0; qb < q1; qb += blockDim.x) {
     const int64_t q = qb + threadIdx.x;
     if (q >= q1) continue;
@@ -684,10 +695,12 @@ __global__ __launch_bounds__(256) void eri_row_kernel(ClassDev cd, BuildArgs ba)
     const PairRec &bra = TR ? mov : fix;
     const PairRec &ket = TR ? fix : mov;
     double v[S::NABCD];
-    if (mixed_low(ba, bra, ket))
+    if (mixed_wave(ba, bra, ket)) {
+      ++nlow;
       small_integrals<float, LA, LB, LC, LD>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, v, tab);
-    else
+    } else {
       small_integrals<double, LA, LB, LC, LD>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, v, tab);
+    }
 
     double s = sfix;
     if (mov.sameAB) s *= 0.5;
@@ -821,6 +834,7 @@ __global__ __launch_bounds__(256) void eri_row_kernel(ClassDev cd, BuildArgs ba)
     }
   }
 
+  count_fp32(ba, nlow);
   // the fixed pair's Coulomb block: reduce over each wave, then over the waves
 #pragma unroll
   for (int k = 0; k < NFIX; ++k) {
