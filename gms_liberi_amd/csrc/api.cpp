@@ -369,6 +369,11 @@ static int read_shells(const char *who, int32_t nsh, int32_t natoms, int32_t num
   const std::string w0 = std::string(who) + ": ";
   if (nsh <= 0 || natoms <= 0 || num_bas <= 0 || mxgtot <= 0)
     return fail(LIBERI_EARG, w0 + "nsh, natoms, num_bas, mxgtot must be positive");
+  // the packed matrices hold num_bas(num_bas+1)/2 elements, a default
+  // integer in the reference's interface (liberi_interface.F90:202-217);
+  // the kernels index them in 32 bits
+  if (num_bas > kMaxBasis)
+    return fail(LIBERI_EARG, w0 + "num_bas above " + std::to_string(kMaxBasis) + " (packed size exceeds 32 bits)");
   if (!ang_mom || !contr_num || !sh_loc || !atom_num || !atom_loc || !exponents || !cs || !cp || !cdc || !cf ||
       !coords)
     return fail(LIBERI_EARG, w0 + "NULL array");

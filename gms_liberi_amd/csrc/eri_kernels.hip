@@ -93,7 +93,8 @@ __host__ __device__ constexpr double cart_scale(int lx, int ly, int lz) {
   return 1.0;
 }
 
-__device__ __forceinline__ int64_t tri(int64_t i, int64_t j) {
+// packed lower-triangle index of (i, j); i, j < kMaxBasis keeps i(i+1) below 2^32
+__device__ __forceinline__ uint32_t tri(uint32_t i, uint32_t j) {
   return i >= j ? ((i * (i + 1)) >> 1) + j : ((j * (j + 1)) >> 1) + i;
 }
 
@@ -235,7 +236,7 @@ __device__ __forceinline__ void decode_quartet(const ClassDev &cd, int64_t q, in
   quartet_of_segment(cd, lo, q, bra, ket);
 }
 
-__device__ __forceinline__ void fock_add(double *F, int64_t i, int64_t j, double v) {
+__device__ __forceinline__ void fock_add(double *F, uint32_t i, uint32_t j, double v) {
   unsafeAtomicAdd(F + tri(i, j), v);
 }
 
@@ -1349,10 +1350,11 @@ __device__ __forceinline__ int gpos_of(int L, int k) {
 // per row tile in LDS rows, or per run of kets sharing a shell -- cut this
 // kernel's atomics 5-9x on C60 but not its time: its atomics overlap its
 // arithmetic, and the LDS rows cost occupancy.)
-// (classes of s, p and d shells up to total angular momentum 6 fit 168
-// VGPRs, i.e. 3 waves per SIMD, without spilling; the hint keeps them there)
+// (classes of s, p and d shells up to total angular momentum 7 fit 168
+// VGPRs, i.e. 3 waves per SIMD, with at most 12 bytes of spill per lane in
+// (dp|dp) and (dd|dp); measured faster than 2 waves without spill)
 template <int LA, int LB, int LC, int LD>
-__attribute__((amdgpu_waves_per_eu(LA <= 2 && LB <= 2 && LC <= 2 && LD <= 2 && LA + LB + LC + LD <= 6 ? 3 : 1, 8)))
+__attribute__((amdgpu_waves_per_eu(LA <= 2 && LB <= 2 && LC <= 2 && LD <= 2 && LA + LB + LC + LD <= 7 ? 3 : 1, 8)))
 __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba) {
   using S = Grp<LA, LB, LC, LD>;
   constexpr int G = S::G, NQ = S::NQ, NR = S::NR, NI = S::NI, NK = S::NK, NE = S::NE, NF = S::NF, NKC = S::NKC;
@@ -1450,13 +1452,15 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
       const int sa = lane / RT, tt = lane - sa * RT, t = t0 + tt;
       if (sa < NQ && t < s_npq[sa]) {
         const int nka = s_nk[sa];
-        const int ip = t / nka, iq = t - ip * nka;
+        // t / nka in FP32: both are small integers, the quotient's fraction
+        // stays >= 0.5/nka away from an integer, far above the rounding error
+        const int ip = (int)(((float)t + 0.5f) * __builtin_amdgcn_rcpf((float)nka)), iq = t - ip * nka;
         const PrimRec P = ba.prims[s_boff[sa] + ip];
         const PrimRec Q = ba.prims[s_koff[sa] + iq];
-        const double pq = P.p + Q.p, ipq = 1.0 / pq;
+        const double rs = nr_rsqrt(P.p + Q.p), ipq = rs * rs;
         const double pqx = P.Px - Q.Px, pqy = P.Py - Q.Py, pqz = P.Pz - Q.Pz;
         const double X = P.p * Q.p * ipq * (pqx * pqx + pqy * pqy + pqz * pqz);
-        const double pref = kTwoPi52 * P.K * Q.K * sqrt(ipq);
+        const double pref = kTwoPi52 * P.K * Q.K * rs;
         double *rec = lds + sa * S::SLOT + KT * S::GS + tt * REC;
 #pragma unroll 1
         for (int r = 0; r < NR; ++r) {
@@ -1468,8 +1472,8 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
         rec[2 * NR] = ipq;
         rec[2 * NR + 1] = Q.p * ipq;
         rec[2 * NR + 2] = P.p * ipq;
-        rec[2 * NR + 3] = 0.5 / P.p;
-        rec[2 * NR + 4] = 0.5 / Q.p;
+        rec[2 * NR + 3] = 0.5 * nr_rcp(P.p);
+        rec[2 * NR + 4] = 0.5 * nr_rcp(Q.p);
         rec[2 * NR + 5] = pqx;
         rec[2 * NR + 6] = pqy;
         rec[2 * NR + 7] = pqz;

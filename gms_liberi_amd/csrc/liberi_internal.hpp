@@ -57,6 +57,9 @@ struct PairRec {
 
 // One primitive pair (a in A, b in B): p = a + b, P, P - A, and
 // K = c_a c_b exp(-ab/p |A-B|^2) / p.
+// largest basis: n(n+1) stays below 2^32, so packed indices fit 32 bits
+constexpr int32_t kMaxBasis = 65535;
+
 struct alignas(64) PrimRec {
   double p, Px, Py, Pz, PAx, PAy, PAz, K;
 };
