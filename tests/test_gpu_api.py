@@ -47,6 +47,21 @@ def test_wrong_size_and_resetup():
 
 
 @pytest.mark.gpu
+def test_basis_beyond_32_bit_packed_size_is_rejected():
+    """num_bas(num_bas+1)/2 is a default integer in the reference's interface;
+    the kernels index packed matrices in 32 bits, so setup refuses
+    num_bas > 65535 with an argument error before touching the arrays."""
+    import dataclasses
+
+    b = Case.load_npz(GOLD / "cfg_water.npz").basis
+    h = liberi.Handle()
+    with pytest.raises(liberi.LiberiError) as e:
+        h.setup(dataclasses.replace(b, num_bas=65536))
+    assert e.value.code == 1
+    h.destroy()
+
+
+@pytest.mark.gpu
 def test_gamess_wrapper_entry_points():
     case = Case.load_npz(GOLD / "ref_h2o_ccpvdz.npz")
     b = case.basis
