@@ -1322,10 +1322,10 @@ struct Grp {
   // p/(p+q), 1/2p, 1/2q, P-Q, P-A, Q-C.  They live in the part of the slot
   // the tables leave free (HRR and density space, written only after the
   // loop); RT of them per slot (a multiple of KT) are made in one pass of the
-  // wave.
+  // wave, one lane per (slot, primitive quartet, root).
   static constexpr int REC = 2 * NR + 14;
   static constexpr int RT0 = (SLOT - KT * GS) / REC;
-  static constexpr int RT1 = RT0 < 64 / NQ ? RT0 : 64 / NQ;
+  static constexpr int RT1 = RT0 < 64 / (NQ * NR) ? RT0 : 64 / (NQ * NR);
   static constexpr int RT = RT1 / KT * KT;
 };
 
@@ -1446,10 +1446,11 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
     for (int k = 0; k < NACC; ++k) acc[k] = 0.0;
 
     // rule records of primitive quartets t0 .. t0+RT-1 of every slot, one
-    // lane per (slot, quartet): the Rys rule, its prefactor, the reciprocals
-    // and the displacements the recursion needs
+    // lane per (slot, quartet, root): the root and weight, and (root 0) the
+    // prefactor, reciprocals and displacements the recursion needs
     auto rules = [&](int t0) {
-      const int sa = lane / RT, tt = lane - sa * RT, t = t0 + tt;
+      const int sa = lane / (RT * NR), rem = lane - sa * (RT * NR), tt = rem / NR, r = rem - tt * NR;
+      const int t = t0 + tt;
       if (sa < NQ && t < s_npq[sa]) {
         const int nka = s_nk[sa];
         // t / nka in FP32: both are small integers, the quotient's fraction
@@ -1462,13 +1463,11 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
         const double X = P.p * Q.p * ipq * (pqx * pqx + pqy * pqy + pqz * pqz);
         const double pref = kTwoPi52 * P.K * Q.K * rs;
         double *rec = lds + sa * S::SLOT + KT * S::GS + tt * REC;
-#pragma unroll 1
-        for (int r = 0; r < NR; ++r) {
-          double ur, wr;
-          rys_root<NR>(X, cd.xswitch, r, ur, wr);
-          rec[r] = ur;
-          rec[NR + r] = wr * pref;
-        }
+        double ur, wr;
+        rys_root<NR>(X, cd.xswitch, r, ur, wr);
+        rec[r] = ur;
+        rec[NR + r] = wr * pref;
+        if (r == 0) {
         rec[2 * NR] = ipq;
         rec[2 * NR + 1] = Q.p * ipq;
         rec[2 * NR + 2] = P.p * ipq;
@@ -1483,6 +1482,7 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
         rec[2 * NR + 11] = Q.PAx;
         rec[2 * NR + 12] = Q.PAy;
         rec[2 * NR + 13] = Q.PAz;
+        }
       }
     };
     // 2-D Rys tables of primitive quartets t0 .. t0+KT-1 (3 NR lanes each:
