@@ -91,7 +91,7 @@ struct ClassState {
 struct liberi_ctx {
   // options
   double schwarz_cut = kRefCutoff, value_cut = kRefCutoff, fp32_tau = 0.0, exact_large_t = 0.0;
-  double prim_cut = 1e-24;
+  double prim_cut = 1e-18;
   int row_tiles = 1;  // 0 never, 1 per class (autotuned or default), 2 wherever they fit
   bool autotune = true;
   // basis
@@ -503,34 +503,65 @@ int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, 
   // ---- primitive-pair screen.  Bound of one primitive pair's share of any
   // integral: B = 2pi^(5/2)^(1/2) |K| (1 + |P-A| + 1/sqrt(p))^(la+lb) (the
   // Rys 2-D factors are polynomials in P-A, P-Q and 1/2p of that degree, the
-  // Boys values are <= 1).  A primitive quartet contributes at most
-  // B_ab * B_cd; pairs with B_ab * max(B) < prim_cut change no integral by
-  // more than prim_cut per primitive quartet, far below the value cutoff, and
-  // are dropped.  Quartet lists and counts are untouched (a pair may end up
-  // with no primitives and then contributes nothing).
+  // Boys values are <= 1); a primitive quartet contributes at most about
+  // B_ab B_cd.  A primitive pair of shell pair ab only ever meets the
+  // primitive pairs of ab's partners: the shell pairs cd with
+  // x_ab x_cd >= cutoff (every quartet test compares that product), and every
+  // pair of the other list in the classes without a quartet test.  With M_ab
+  // the largest B among those partners, primitive pairs with B M_ab <
+  // prim_cut change no integral by more than prim_cut per primitive quartet,
+  // far below the value cutoff, and are dropped.  Quartet lists and counts are
+  // untouched (a pair may end up with no primitives and then contributes
+  // nothing).
   if (h->prim_cut > 0.0) {
-    std::vector<double> bound(prims.size());
-    double bmax = 0.0;
-    std::vector<int> pair_lsum(pairs.size());
+    const size_t np = pairs.size();
+    std::vector<double> bound(prims.size()), pmax(np, 0.0);
+    std::vector<int> pair_pc(np);
     for (int pc = 0; pc < kNumPairClasses; ++pc)
-      for (int32_t p : class_pairs[pc]) pair_lsum[p] = kPairL[pc][0] + kPairL[pc][1];
-    for (size_t ip = 0; ip < pairs.size(); ++ip) {
+      for (int32_t p : class_pairs[pc]) pair_pc[p] = pc;
+    for (size_t ip = 0; ip < np; ++ip) {
       const PairRec &r = pairs[ip];
+      const int lsum = kPairL[pair_pc[ip]][0] + kPairL[pair_pc[ip]][1];
       for (int k = 0; k < r.nprim; ++k) {
         const PrimRec &q = prims[r.prim_off + k];
         const double pa = std::sqrt(q.PAx * q.PAx + q.PAy * q.PAy + q.PAz * q.PAz);
-        const double b = std::sqrt(kTwoPi52) * std::fabs(q.K) * std::pow(1.0 + pa + 1.0 / std::sqrt(q.p), pair_lsum[ip]);
+        const double b = std::sqrt(kTwoPi52) * std::fabs(q.K) * std::pow(1.0 + pa + 1.0 / std::sqrt(q.p), lsum);
         bound[r.prim_off + k] = b;
-        bmax = std::max(bmax, b);
+        pmax[ip] = std::max(pmax[ip], b);
       }
     }
+    // partners through the quartet test: pairs by Schwarz value descending,
+    // running maximum of their bounds
+    std::vector<int32_t> byx(np);
+    for (size_t ip = 0; ip < np; ++ip) byx[ip] = (int32_t)ip;
+    std::sort(byx.begin(), byx.end(), [&](int32_t a, int32_t b) { return pair_x[a] > pair_x[b]; });
+    std::vector<double> xdesc(np), runmax(np);
+    for (size_t k = 0; k < np; ++k) {
+      xdesc[k] = pair_x[byx[k]];
+      runmax[k] = std::max(k ? runmax[k - 1] : 0.0, pmax[byx[k]]);
+    }
+    // partners in the classes without a quartet test: the other list's pairs
+    std::vector<double> class_max(kNumPairClasses, 0.0), untested(kNumPairClasses, 0.0);
+    for (size_t ip = 0; ip < np; ++ip) class_max[pair_pc[ip]] = std::max(class_max[pair_pc[ip]], pmax[ip]);
+    for (int c = 0; c < kNumClasses; ++c)
+      if (kClasses[c].cmp == CMP_NONE) {
+        untested[kClasses[c].bra] = std::max(untested[kClasses[c].bra], class_max[kClasses[c].ket]);
+        untested[kClasses[c].ket] = std::max(untested[kClasses[c].ket], class_max[kClasses[c].bra]);
+      }
     std::vector<PrimRec> kept;
     kept.reserve(prims.size());
-    for (size_t ip = 0; ip < pairs.size(); ++ip) {
+    for (size_t ip = 0; ip < np; ++ip) {
       PairRec &r = pairs[ip];
+      double m = untested[pair_pc[ip]];
+      if (pair_x[ip] > 0.0) {
+        // x_cd >= cutoff / x_ab, with a margin for the rounding of the quotient
+        const double t = h->schwarz_cut / pair_x[ip] * (1.0 - 1e-12);
+        const size_t n = (size_t)(std::upper_bound(xdesc.begin(), xdesc.end(), t, std::greater<double>()) - xdesc.begin());
+        if (n > 0) m = std::max(m, runmax[n - 1]);
+      }
       const int32_t off = (int32_t)kept.size();
       for (int k = 0; k < r.nprim; ++k)
-        if (bound[r.prim_off + k] * bmax >= h->prim_cut) kept.push_back(prims[r.prim_off + k]);
+        if (bound[r.prim_off + k] * m >= h->prim_cut) kept.push_back(prims[r.prim_off + k]);
       r.prim_off = off;
       r.nprim = (int32_t)kept.size() - off;
     }

@@ -110,8 +110,10 @@ const char *liberi_c_last_error(void);
                                        count for its Rys classes); 1: only where exact to double */
 #define LIBERI_OPT_CLASS_TIMING 5   /* 1: time every class launch with HIP events (stats.class_ms) */
 #define LIBERI_OPT_ONLY_CLASS 6     /* >= 0: build only this class (diagnostics); -1 (default): all */
-#define LIBERI_OPT_PRIM_CUTOFF 7    /* drop primitive pairs whose share of any integral is below this
-                                       bound (default 1e-24; 0 keeps every primitive pair) */
+#define LIBERI_OPT_PRIM_CUTOFF 7    /* drop primitive pairs whose share of any integral they can
+                                       enter (with the pairs that pass the quartet test with
+                                       theirs) is bounded below this (default 1e-18; 0 keeps every
+                                       primitive pair) */
 #define LIBERI_OPT_ROW_TILES 8      /* digestion: row tiles accumulate the exchange blocks on chip
                                        per tile of quartets sharing one shell pair; direct digestion
                                        adds every element of every quartet to F atomically.  1

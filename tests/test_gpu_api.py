@@ -47,6 +47,36 @@ def test_wrong_size_and_resetup():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("name", ["cfg_ethane", "cfg_cn_chain"])
+def test_primitive_pair_screen_is_below_summation_noise(name):
+    """Primitive pairs dropped at setup (their bound times the largest bound
+    among the shell pairs they can meet through the quartet test is below the
+    cutoff) move F by no more than summation-order noise: within 1e-12 of the
+    largest element of the build that keeps every primitive pair; quartet
+    counts are the same."""
+    case = Case.load_npz(GOLD / f"{name}.npz")
+    h0 = liberi.Handle()
+    h0.set_option(liberi.OPT_PRIM_CUTOFF, 0.0)
+    h0.setup(case.basis)
+    f0 = h0.fock_build(case.density)
+    st0 = h0.stats()
+    h0.destroy()
+    h = liberi.Handle()
+    assert h.get_option(liberi.OPT_PRIM_CUTOFF) == 1e-18
+    h.setup(case.basis)
+    f = h.fock_build(case.density)
+    st = h.stats()
+    h.destroy()
+    assert st0["prim_pairs_dropped"] == 0
+    assert np.array_equal(st["quartets"], st0["quartets"])
+    assert st["prim_pairs"] + st["prim_pairs_dropped"] == st0["prim_pairs"]
+    if name == "cfg_cn_chain":
+        assert st["prim_pairs_dropped"] > 0
+    scale = max(1.0, float(np.max(np.abs(f0))))
+    assert np.max(np.abs(f - f0)) <= 1e-12 * scale
+
+
+@pytest.mark.gpu
 def test_basis_beyond_32_bit_packed_size_is_rejected():
     """num_bas(num_bas+1)/2 is a default integer in the reference's interface;
     the kernels index packed matrices in 32 bits, so setup refuses
