@@ -1274,6 +1274,92 @@ __host__ __device__ constexpr int grp_parts(int nl, int nr, int slot) {
   return best;
 }
 
+// LDS cycles of one wave access to the distinct doubles at addresses
+// a[0..n) (32 banks of 4 bytes, double d on banks 2d and 2d+1 mod 32): the
+// largest number of addresses that share a bank pair
+constexpr int lds_cycles(const int *a, int n) {
+  int cnt[16] = {};
+  for (int i = 0; i < n; ++i) ++cnt[a[i] % 16];
+  int m = 0;
+  for (int k = 0; k < 16; ++k) m = cnt[k] > m ? cnt[k] : m;
+  return m;
+}
+
+// Padding of the 2-D table stride (gsz0 + gp doubles) and of the slot
+// (+ sp doubles), chosen to minimise the LDS cycles of the primitive loop:
+// the accumulation reads (one instruction reads the same table element
+// offset of every slot, root part and row of the lanes) and the table writes
+// (one element of every (slot, quartet, root, direction) table), weighted by
+// how often each runs per KT primitive quartets.  Candidates that lose a
+// table pass's quartets, leave 64 KB, or push a slot set that fitted three
+// waves per SIMD out of that budget are skipped.  Returns gp * 16 + sp
+// (0 for the lane-per-quartet classes, which have no slots, and for strided
+// maps of more than 128 components; strided maps pad the slot only).
+template <int LA, int LB, int LC, int LD>
+constexpr int grp_pad(int map, int p, int nq, int g, int nr, int gsz0, int bufp, int db) {
+  constexpr int LAB = LA + LB, LCD = LC + LD, NI = LAB + 1, NK = LCD + 1;
+  constexpr Range<LA, LAB> re{};
+  constexpr Range<LC, LCD> rf{};
+  constexpr int NE = re.n, NF = rf.n;
+  if (NE * NF <= 100 && hrr_max_buffer(LA, LB, LC, LD) <= 100) return 0;
+  if (map == 0 && NE * NF > 128) return 0;  // (keeps the search within the compiler's step budget)
+  int best = 0, best_cost = -1, kt0 = 0;
+  bool fit3 = false;
+  for (int gp = 0; gp < (map == 0 ? 1 : 16); ++gp)
+    for (int sp = 0; sp < 16; ++sp) {
+      const int gsz = gsz0 + gp, gs = 3 * nr * gsz;
+      const int uni = 2 * gs > 2 * bufp ? 2 * gs : 2 * bufp, slot = uni + db + sp;
+      int kt = g / (3 * nr) < uni / gs ? g / (3 * nr) : uni / gs;
+      kt = kt < 1 ? 1 : kt;
+      if (gp == 0 && sp == 0) {
+        kt0 = kt;
+        fit3 = nq * slot * 8 <= kGroupLdsTarget;
+      }
+      if (kt != kt0 || nq * slot * 8 > 64 * 1024 || (fit3 && nq * slot * 8 > kGroupLdsTarget)) continue;
+      int addr[64] = {};
+      int cost = 0;
+      // accumulation reads
+      if (map == 1 || map == 2) {
+        const int nx = map == 1 ? NI : NK, xs = map == 1 ? NK : 1, nj = map == 1 ? NK : NI;
+        int n = 0;
+        for (int q = 0; q < nq; ++q)
+          for (int pp = 0; pp < p; ++pp)
+            for (int x = 0; x < nx; ++x) addr[n++] = q * slot + pp * gsz + x * xs;
+        cost += kt * 3 * nj * ((nr + p - 1) / p) * lds_cycles(addr, n);
+      } else {
+        for (int k = 0; k * g < NE * NF; ++k)
+          for (int dim = 0; dim < 3; ++dim) {
+            bool seen[64] = {};  // table offsets ec * NK + fc < NI * NK <= 64
+            for (int l = 0; l < g; ++l) {
+              const int c = l + g * k;
+              if (c >= NE * NF) break;
+              const int e = c / NF, f = c % NF;
+              const int ec = dim == 0 ? re.lx[e] : (dim == 1 ? re.ly[e] : re.lz[e]);
+              const int fc = dim == 0 ? rf.lx[f] : (dim == 1 ? rf.ly[f] : rf.lz[f]);
+              seen[ec * NK + fc] = true;
+            }
+            int n = 0;
+            for (int q = 0; q < nq; ++q)
+              for (int o = 0; o < NI * NK && o < 64; ++o)
+                if (seen[o] && n < 64) addr[n++] = q * slot + o;
+            cost += kt * nr * lds_cycles(addr, n);
+          }
+      }
+      // table writes
+      {
+        int n = 0;
+        for (int q = 0; q < nq; ++q)
+          for (int m = 0; m < kt * 3 * nr && n < 64; ++m) addr[n++] = q * slot + m * gsz;
+        cost += NI * NK * lds_cycles(addr, n);
+      }
+      if (best_cost < 0 || cost < best_cost) {
+        best_cost = cost;
+        best = gp * 16 + sp;
+      }
+    }
+  return best;
+}
+
 template <int LA, int LB, int LC, int LD>
 struct Grp {
   static constexpr int LAB = LA + LB, LCD = LC + LD;
@@ -1282,7 +1368,6 @@ struct Grp {
   static constexpr int NE = nrange(LA, LAB), NF = nrange(LC, LCD);
   static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   static constexpr int NABCD = NA * NB * NC * ND;
-  static constexpr int GS = 3 * NR * NI * NK;
   static constexpr int BUF = hrr_max_buffer(LA, LB, LC, LD);
   static constexpr int NAB = NA * NB, NCD = NC * ND, NAC = NA * NC, NBD = NB * ND, NAD = NA * ND, NBC = NB * NC;
   static constexpr int DB = NAB + NCD + NAC + NBD + NAD + NBC;
@@ -1291,10 +1376,11 @@ struct Grp {
   static constexpr int KS = odd_at_least(ket_row_max(LC, LD));
   // buffers of the HRR: bra phase [e][b][f] (stride NF), ket phase [ab][KS]
   static constexpr int BUFP = grp_buffer(LA, LB, LC, LD, KS);
-  // two G-table buffers (primitive loop, double-buffered), dead once the loop
-  // ends: the HRR buffers alias them
-  static constexpr int UNI = 2 * GS > 2 * BUFP ? 2 * GS : 2 * BUFP;
-  static constexpr int SLOT = UNI + DB;
+  // slot before padding: table buffers (at least two), dead once the
+  // primitive loop ends, aliased by the HRR buffers; the density blocks
+  static constexpr int GSZ0 = NI * NK;
+  static constexpr int UNI0 = 2 * 3 * NR * GSZ0 > 2 * BUFP ? 2 * 3 * NR * GSZ0 : 2 * BUFP;
+  static constexpr int SLOT0 = UNI0 + DB;
   // lane map of the primitive loop.  MAP 1: a lane per row e of (e0|f0)
   // accumulating all its columns f; MAP 2: a lane per column f accumulating
   // all rows e; in both the roots are split over P lanes (partial sums added
@@ -1306,12 +1392,19 @@ struct Grp {
   // same for any map, so fewer quartets per wave cost more than the saved
   // reads: e.g. (ps|dd), rows of 3).
   static constexpr int NL = NE >= NF ? NE : NF;
-  static constexpr int P0 = NL <= 64 ? grp_parts(NL, NR, SLOT) : 0;
+  static constexpr int P0 = NL <= 64 ? grp_parts(NL, NR, SLOT0) : 0;
   static constexpr int P = P0 && 64 / (NL * P0) >= 64 / GroupSize<LA, LB, LC, LD>::G ? P0 : 0;
   static constexpr int MAP = P == 0 ? 0 : (NE >= NF ? 1 : 2);
   static constexpr int G = MAP ? NL * P : GroupSize<LA, LB, LC, LD>::G;
   static constexpr int NQ = 64 / G;
   static constexpr int NKC = (NE * NF + G - 1) / G;
+  // table stride (one 2-D table of NI x NK, padded) and slot, padded
+  // against LDS bank conflicts
+  static constexpr int PAD = grp_pad<LA, LB, LC, LD>(MAP, P, NQ, G, NR, GSZ0, BUFP, DB);
+  static constexpr int GSZ = GSZ0 + PAD / 16;
+  static constexpr int GS = 3 * NR * GSZ;
+  static constexpr int UNI = 2 * GS > 2 * BUFP ? 2 * GS : 2 * BUFP;
+  static constexpr int SLOT = UNI + DB + PAD % 16;
   static constexpr bool fits = SLOT * NQ * 8 <= 64 * 1024;
   // primitive quartets whose 2-D tables one pass of the table lanes makes
   // (3 NR lanes each), held in the HRR space of the slot
@@ -1359,7 +1452,7 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
   using S = Grp<LA, LB, LC, LD>;
   constexpr int G = S::G, NQ = S::NQ, NR = S::NR, NI = S::NI, NK = S::NK, NE = S::NE, NF = S::NF, NKC = S::NKC;
   constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND, NABCD = S::NABCD;
-  constexpr int GSZ = NI * NK;
+  constexpr int GSZ = S::GSZ;  // stride of the 2-D tables (padded against bank conflicts)
   constexpr int RT = S::RT, REC = S::REC, KT = S::KT;
   static_assert(RT >= KT && KT * S::GS <= S::UNI, "no room for the tables and rule records");
   __shared__ double lds[NQ * S::SLOT];
