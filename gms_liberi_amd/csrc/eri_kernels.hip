@@ -124,14 +124,15 @@ __device__ __forceinline__ float nr_rcp(float x) {
 // large-argument Boys values F_m = (2m-1)!!/(2X)^m sqrt(pi/4X) that the
 // reference uses past its switch points.
 // tab: the N-root part of the table (rys_table + rys_offset[N], or a copy of
-// its first intervals in LDS)
+// its first intervals in LDS), istride doubles per interval
 template <int N>
 __device__ __forceinline__ void rys_rule(double X, double xswitch, double (&u)[N], double (&w)[N],
-                                         const double *tab = rys_table + rys_offset[N]) {
+                                         const double *tab = rys_table + rys_offset[N],
+                                         int istride = 2 * N * (RYS_DEG + 1)) {
   if (X < xswitch) {
     const int j = (int)(X * RYS_INV_H);
     const double s = X * (2.0 * RYS_INV_H) - (double)(2 * j + 1);
-    const double *c = tab + j * (2 * N * (RYS_DEG + 1));
+    const double *c = tab + j * istride;
 #pragma unroll
     for (int r = 0; r < 2 * N; ++r) {
       const double *cr = c + r * (RYS_DEG + 1);
@@ -357,7 +358,8 @@ template <class T, int LA, int LB, int LC, int LD>
 __device__ __forceinline__ void small_integrals(const PairRec &bra, const PairRec &ket, const PrimRec *prims,
                                                 double xswitch, double cut,
                                                 double (&v)[Small<LA, LB, LC, LD>::NABCD],
-                                                const double *tab = rys_table + rys_offset[Small<LA, LB, LC, LD>::NR]) {
+                                                const double *tab = rys_table + rys_offset[Small<LA, LB, LC, LD>::NR],
+                                                int istride = 2 * Small<LA, LB, LC, LD>::NR * (RYS_DEG + 1)) {
   using S = Small<LA, LB, LC, LD>;
   constexpr int NE = S::NE, NF = S::NF, NR = S::NR, NI = S::LAB + 1, NK = S::LCD + 1;
   constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND;
@@ -379,7 +381,7 @@ __device__ __forceinline__ void small_integrals(const PairRec &bra, const PairRe
       const double X = rho * (pqx * pqx + pqy * pqy + pqz * pqz);
       const double pref = kTwoPi52 * P.K * Q.K * rs;
       double u[NR], w[NR];
-      rys_rule<NR>(X, xswitch, u, w, tab);
+      rys_rule<NR>(X, xswitch, u, w, tab, istride);
       const T tp = (T)P.p, tq = (T)Q.p, tipq = (T)ipq;
       const T hp = (T)0.5 * nr_rcp(tp), hq = (T)0.5 * nr_rcp(tq);
       const T tpqx = (T)pqx, tpqy = (T)pqy, tpqz = (T)pqz;
@@ -597,14 +599,20 @@ __device__ __forceinline__ void lds_add(double *p, double v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Doubles of the N-root table that the rule reads below xswitch (its first
-// ceil(xswitch / h) intervals).
-__host__ __device__ inline int64_t rys_table_used(int n, double xswitch) {
+// Intervals of the N-root table that the rule reads below xswitch (the
+// first ceil(xswitch / h)), and the doubles their LDS copy takes: the copy
+// pads each interval to an odd stride, so lanes reading different intervals
+// spread over the LDS banks.
+__host__ __device__ inline int64_t rys_table_intervals(int n, double xswitch) {
   int64_t nint = (int64_t)ceil(xswitch * RYS_INV_H);
   const int64_t cap = n == 1 ? 78 : n == 2 ? 93 : n == 3 ? 107 : n == 4 ? 120 : n == 5 ? 132 : n == 6 ? 144 : 155;
   if (nint > cap) nint = cap;
   if (nint < 0) nint = 0;
-  return nint * 2 * n * (RYS_DEG + 1);
+  return nint;
+}
+__host__ __device__ constexpr int rys_lds_stride(int n) { return 2 * n * (RYS_DEG + 1) + 1; }
+__host__ __device__ inline int64_t rys_table_used(int n, double xswitch) {
+  return rys_table_intervals(n, xswitch) * rys_lds_stride(n);
 }
 
 // Shape of a row tile of class (LA LB | LC LD).  The tile fixes one shell
@@ -654,12 +662,14 @@ __global__ __launch_bounds__(256) void eri_row_kernel(ClassDev cd, BuildArgs ba)
   double *jfix_s = lds, *rows = lds + NFIX;
   const int ncols = (int)row_columns<LA, LB, LC, LD, TR>(ba.ncol);
   const int off2 = R::M1 == R::M2 ? 0 : ba.ncol[R::M1];
-  // the class's Rys table below the switch, staged once per workgroup
+  // the class's Rys table below the switch, staged once per workgroup with
+  // intervals padded to an odd stride
   double *tab = rows + NROW * ncols;
   {
-    const int64_t ntab = rys_table_used(S::NR, cd.xswitch);
+    constexpr int IN = 2 * S::NR * (RYS_DEG + 1), LS = rys_lds_stride(S::NR);
+    const int ntab = (int)rys_table_intervals(S::NR, cd.xswitch) * IN;
     const double *src = rys_table + rys_offset[S::NR];
-    for (int64_t i = threadIdx.x; i < ntab; i += blockDim.x) tab[i] = src[i];
+    for (int i = threadIdx.x; i < ntab; i += blockDim.x) tab[(i / IN) * LS + i % IN] = src[i];
   }
 
   const int tile = cd.t_begin + (int)blockIdx.x;
@@ -698,9 +708,11 @@ __global__ __launch_bounds__(256) void eri_row_kernel(ClassDev cd, BuildArgs ba)
     double v[S::NABCD];
     if (mixed_wave(ba, bra, ket)) {
       ++nlow;
-      small_integrals<float, LA, LB, LC, LD>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, v, tab);
+      small_integrals<float, LA, LB, LC, LD>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, v, tab,
+                                             rys_lds_stride(S::NR));
     } else {
-      small_integrals<double, LA, LB, LC, LD>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, v, tab);
+      small_integrals<double, LA, LB, LC, LD>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, v, tab,
+                                              rys_lds_stride(S::NR));
     }
 
     double s = sfix;
