@@ -373,8 +373,14 @@ __device__ __forceinline__ void small_integrals(const PairRec &bra, const PairRe
 
   for (int ip = 0; ip < bra.nprim; ++ip) {
     const PrimRec P = prims[bra.prim_off + ip];
+    // (ss|ss): the next ket primitive is loaded one iteration ahead, its
+    // latency hidden behind this iteration's arithmetic (in the other classes
+    // the extra registers cost more occupancy than the latency it hides)
+    constexpr bool ahead = LA + LB + LC + LD == 0;
+    PrimRec Qn = prims[ket.prim_off];
     for (int iq = 0; iq < ket.nprim; ++iq) {
-      const PrimRec Q = prims[ket.prim_off + iq];
+      const PrimRec Q = ahead ? Qn : prims[ket.prim_off + iq];
+      if (ahead && iq + 1 < ket.nprim) Qn = prims[ket.prim_off + iq + 1];
       const double rs = nr_rsqrt(P.p + Q.p), ipq = rs * rs;  // 1/sqrt(p+q), 1/(p+q)
       const double pqx = P.Px - Q.Px, pqy = P.Py - Q.Py, pqz = P.Pz - Q.Pz;
       const double rho = P.p * Q.p * ipq;
