@@ -1181,6 +1181,76 @@ __global__ __launch_bounds__((Coop<LA, LB, LC, LD>::NT)) void eri_coop_kernel(Cl
 // quartet, two HRR buffers, the six density blocks.  Loop bounds are uniform
 // over the wave (primitive loop to the wave's longest quartet, others
 // predicated), so the single-wave barriers are cheap and always matched.
+// components of angular momenta l0..L (runtime L >= l0 - 1)
+__device__ __forceinline__ int nrange_rt(int l0, int L) {
+  return ((L + 1) * (L + 2) * (L + 3) - l0 * (l0 + 1) * (l0 + 2)) / 6;
+}
+
+// The HRR levels in a second form, for levels whose columns (bra) or rows
+// (ket) are fewer than the lanes: lanes over (component, column) pairs.  A
+// lane decodes its component once and finds the positions of its three
+// neighbours one unit higher; the loop over the level's new cartesian
+// components stays compile-time.  Same arithmetic as the one-lane-per-column
+// form below; grp_hrr_cost picks the form with fewer issued instructions.
+template <int LA, int LAB, int J, int NF, int G, int OS>
+__device__ __forceinline__ void grp_bra_level_af(const double *__restrict__ in, double *__restrict__ out, int gl,
+                                                 double abx, double aby, double abz) {
+  constexpr Range<J, J> rb{};
+  constexpr int na = nrange(LA, LAB - J), nbm = ncart(J - 1), nb = ncart(J);
+  for (int t = gl; t < na * NF; t += G) {
+    const int a = t / NF, f = t - a * NF;
+    int L, ax, ay, az;
+    range_decode(LA, a, L, ax, ay, az);
+    // positions of a + e_x, a + e_y, a + e_z in range(LA, LAB - J + 1)
+    const int base = nrange_rt(LA, L), s1 = ay + az + 1;
+    const int px = base + cidx(ax, ay, az), py = base + (s1 * (s1 + 1)) / 2 + az, pz = py + 1;
+    const double *i0 = in + a * nbm * NF + f;
+    const double *ix = in + px * nbm * NF + f, *iy = in + py * nbm * NF + f, *iz = in + pz * nbm * NF + f;
+    double *o = out + a * nb * OS + f;
+#pragma unroll
+    for (int b = 0; b < nb; ++b) {
+      const int bx = rb.lx[b], by = rb.ly[b], bz = rb.lz[b];
+      const int dir = bx ? 0 : (by ? 1 : 2);
+      const int bmi = cidx(bx - (dir == 0), by - (dir == 1), bz - (dir == 2));
+      const double *ip = dir == 0 ? ix : (dir == 1 ? iy : iz);
+      const double ab = dir == 0 ? abx : (dir == 1 ? aby : abz);
+      o[b * OS] = ip[bmi * NF] + ab * i0[bmi * NF];
+    }
+  }
+}
+template <int LC, int LCD, int J, int NAB, int G, int KS>
+__device__ __forceinline__ void grp_ket_level_rc(const double *__restrict__ in, double *__restrict__ out, int gl,
+                                                 double cdx, double cdy, double cdz) {
+  constexpr Range<J, J> rd{};
+  constexpr int nc = nrange(LC, LCD - J), ndm = ncart(J - 1), nd = ncart(J);
+  for (int t = gl; t < NAB * nc; t += G) {
+    const int ab = t / nc, c = t - ab * nc;
+    int L, cx, cy, cz;
+    range_decode(LC, c, L, cx, cy, cz);
+    const int base = nrange_rt(LC, L), s1 = cy + cz + 1;
+    const int px = base + cidx(cx, cy, cz), py = base + (s1 * (s1 + 1)) / 2 + cz, pz = py + 1;
+    const double *ip = in + ab * KS;
+    const double *i0 = ip + c * ndm, *ix = ip + px * ndm, *iy = ip + py * ndm, *iz = ip + pz * ndm;
+    double *op = out + ab * KS + c * nd;
+#pragma unroll
+    for (int d = 0; d < nd; ++d) {
+      const int dx = rd.lx[d], dy = rd.ly[d], dz = rd.lz[d];
+      const int dir = dx ? 0 : (dy ? 1 : 2);
+      const int dmi = cidx(dx - (dir == 0), dy - (dir == 1), dz - (dir == 2));
+      const double *i1 = dir == 0 ? ix : (dir == 1 ? iy : iz);
+      const double cdv = dir == 0 ? cdx : (dir == 1 ? cdy : cdz);
+      op[d] = i1[dmi] + cdv * i0[dmi];
+    }
+  }
+}
+// Whether the pair form issues fewer instructions (estimate: 4 per output,
+// 20 per decoded component) for a level of `lines` columns (bra) or rows
+// (ket), m components per line and n new cartesian components each, on g
+// lanes: one lane per line does m n outputs, the pair form n per position
+__host__ __device__ constexpr bool grp_pair_form(int lines, int m, int n, int g) {
+  return ((lines * m + g - 1) / g) * (n * 4 + 20) < ((lines + g - 1) / g) * (m * n * 4);
+}
+
 template <int LA, int LB, int LC, int LD>
 struct GroupSize {
   static constexpr int NEF = nrange(LA, LA + LB) * nrange(LC, LC + LD);
@@ -1710,32 +1780,50 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
     // constant and the loads of one level issue back to back.
     double *in = buf0, *out = buf1;
     if constexpr (LB >= 1) {
-      grp_bra_level<LA, S::LAB, 1, NF, G, (LB == 1 ? S::KS : NF)>(in, out, gl, bra.ABx, bra.ABy, bra.ABz);
+      if constexpr (grp_pair_form(NF, nrange(LA, S::LAB - 1), ncart(1), G))
+        grp_bra_level_af<LA, S::LAB, 1, NF, G, (LB == 1 ? S::KS : NF)>(in, out, gl, bra.ABx, bra.ABy, bra.ABz);
+      else
+        grp_bra_level<LA, S::LAB, 1, NF, G, (LB == 1 ? S::KS : NF)>(in, out, gl, bra.ABx, bra.ABy, bra.ABz);
       wave_sync();
       double *t = in; in = out; out = t;
     }
     if constexpr (LB >= 2) {
-      grp_bra_level<LA, S::LAB, 2, NF, G, (LB == 2 ? S::KS : NF)>(in, out, gl, bra.ABx, bra.ABy, bra.ABz);
+      if constexpr (grp_pair_form(NF, nrange(LA, S::LAB - 2), ncart(2), G))
+        grp_bra_level_af<LA, S::LAB, 2, NF, G, (LB == 2 ? S::KS : NF)>(in, out, gl, bra.ABx, bra.ABy, bra.ABz);
+      else
+        grp_bra_level<LA, S::LAB, 2, NF, G, (LB == 2 ? S::KS : NF)>(in, out, gl, bra.ABx, bra.ABy, bra.ABz);
       wave_sync();
       double *t = in; in = out; out = t;
     }
     if constexpr (LB >= 3) {
-      grp_bra_level<LA, S::LAB, 3, NF, G, S::KS>(in, out, gl, bra.ABx, bra.ABy, bra.ABz);
+      if constexpr (grp_pair_form(NF, nrange(LA, S::LAB - 3), ncart(3), G))
+        grp_bra_level_af<LA, S::LAB, 3, NF, G, S::KS>(in, out, gl, bra.ABx, bra.ABy, bra.ABz);
+      else
+        grp_bra_level<LA, S::LAB, 3, NF, G, S::KS>(in, out, gl, bra.ABx, bra.ABy, bra.ABz);
       wave_sync();
       double *t = in; in = out; out = t;
     }
     if constexpr (LD >= 1) {
-      grp_ket_level<LC, S::LCD, 1, S::NAB, G, S::KS>(in, out, gl, ket.ABx, ket.ABy, ket.ABz);
+      if constexpr (grp_pair_form(S::NAB, nrange(LC, S::LCD - 1), ncart(1), G))
+        grp_ket_level_rc<LC, S::LCD, 1, S::NAB, G, S::KS>(in, out, gl, ket.ABx, ket.ABy, ket.ABz);
+      else
+        grp_ket_level<LC, S::LCD, 1, S::NAB, G, S::KS>(in, out, gl, ket.ABx, ket.ABy, ket.ABz);
       wave_sync();
       double *t = in; in = out; out = t;
     }
     if constexpr (LD >= 2) {
-      grp_ket_level<LC, S::LCD, 2, S::NAB, G, S::KS>(in, out, gl, ket.ABx, ket.ABy, ket.ABz);
+      if constexpr (grp_pair_form(S::NAB, nrange(LC, S::LCD - 2), ncart(2), G))
+        grp_ket_level_rc<LC, S::LCD, 2, S::NAB, G, S::KS>(in, out, gl, ket.ABx, ket.ABy, ket.ABz);
+      else
+        grp_ket_level<LC, S::LCD, 2, S::NAB, G, S::KS>(in, out, gl, ket.ABx, ket.ABy, ket.ABz);
       wave_sync();
       double *t = in; in = out; out = t;
     }
     if constexpr (LD >= 3) {
-      grp_ket_level<LC, S::LCD, 3, S::NAB, G, S::KS>(in, out, gl, ket.ABx, ket.ABy, ket.ABz);
+      if constexpr (grp_pair_form(S::NAB, nrange(LC, S::LCD - 3), ncart(3), G))
+        grp_ket_level_rc<LC, S::LCD, 3, S::NAB, G, S::KS>(in, out, gl, ket.ABx, ket.ABy, ket.ABz);
+      else
+        grp_ket_level<LC, S::LCD, 3, S::NAB, G, S::KS>(in, out, gl, ket.ABx, ket.ABy, ket.ABz);
       wave_sync();
       double *t = in; in = out; out = t;
     }
