@@ -354,7 +354,7 @@ __device__ __forceinline__ void count_fp32(const BuildArgs &ba, unsigned n) {
 // arithmetic type of the 2-D integrals, the primitive accumulation and the
 // HRR; the Rys argument, roots and weights are always FP64 (they stand where
 // the reference evaluates its Boys function), and so is the result.
-template <class T, int LA, int LB, int LC, int LD>
+template <class T, int LA, int LB, int LC, int LD, bool KET_OUTER = false>
 __device__ __forceinline__ void small_integrals(const PairRec &bra, const PairRec &ket, const PrimRec *prims,
                                                 double xswitch, double cut,
                                                 double (&v)[Small<LA, LB, LC, LD>::NABCD],
@@ -371,42 +371,56 @@ __device__ __forceinline__ void small_integrals(const PairRec &bra, const PairRe
 #pragma unroll
   for (int i = 0; i < NE * NF; ++i) acc[i] = (T)0;
 
-  for (int ip = 0; ip < bra.nprim; ++ip) {
-    const PrimRec P = prims[bra.prim_off + ip];
-    // (ss|ss): the next ket primitive is loaded one iteration ahead, its
-    // latency hidden behind this iteration's arithmetic (in the other classes
-    // the extra registers cost more occupancy than the latency it hides)
-    constexpr bool ahead = LA + LB + LC + LD == 0;
-    PrimRec Qn = prims[ket.prim_off];
+  // one primitive quartet into acc
+  auto prim = [&](const PrimRec &P, const PrimRec &Q) {
+    const double rs = nr_rsqrt(P.p + Q.p), ipq = rs * rs;  // 1/sqrt(p+q), 1/(p+q)
+    const double pqx = P.Px - Q.Px, pqy = P.Py - Q.Py, pqz = P.Pz - Q.Pz;
+    const double rho = P.p * Q.p * ipq;
+    const double X = rho * (pqx * pqx + pqy * pqy + pqz * pqz);
+    const double pref = kTwoPi52 * P.K * Q.K * rs;
+    double u[NR], w[NR];
+    rys_rule<NR>(X, xswitch, u, w, tab, istride);
+    const T tp = (T)P.p, tq = (T)Q.p, tipq = (T)ipq;
+    const T hp = (T)0.5 * nr_rcp(tp), hq = (T)0.5 * nr_rcp(tq);
+    const T tpqx = (T)pqx, tpqy = (T)pqy, tpqz = (T)pqz;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const T t2 = (T)u[r];
+      const T b00 = (T)0.5 * t2 * tipq;
+      const T b10 = hp - hp * tq * t2 * tipq;
+      const T b01 = hq - hq * tp * t2 * tipq;
+      const T fa = tq * tipq * t2, fc = tp * tipq * t2;
+      T gx[NI][NK], gy[NI][NK], gz[NI][NK];
+      rys_2d<NI, NK>(gx, (T)P.PAx - fa * tpqx, (T)Q.PAx + fc * tpqx, b10, b01, b00, (T)1);
+      rys_2d<NI, NK>(gy, (T)P.PAy - fa * tpqy, (T)Q.PAy + fc * tpqy, b10, b01, b00, (T)1);
+      rys_2d<NI, NK>(gz, (T)P.PAz - fa * tpqz, (T)Q.PAz + fc * tpqz, b10, b01, b00, (T)(w[r] * pref));
+#pragma unroll
+      for (int e = 0; e < NE; ++e)
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+          acc[e * NF + f] += gx[re.lx[e]][rf.lx[f]] * gy[re.ly[e]][rf.ly[f]] * gz[re.lz[e]][rf.lz[f]];
+    }
+  };
+  if constexpr (KET_OUTER) {
+    // the inner loop walks the bra pair's primitives (the pair a row tile
+    // keeps fixed: every lane then reads the same records)
     for (int iq = 0; iq < ket.nprim; ++iq) {
-      const PrimRec Q = ahead ? Qn : prims[ket.prim_off + iq];
-      if (ahead && iq + 1 < ket.nprim) Qn = prims[ket.prim_off + iq + 1];
-      const double rs = nr_rsqrt(P.p + Q.p), ipq = rs * rs;  // 1/sqrt(p+q), 1/(p+q)
-      const double pqx = P.Px - Q.Px, pqy = P.Py - Q.Py, pqz = P.Pz - Q.Pz;
-      const double rho = P.p * Q.p * ipq;
-      const double X = rho * (pqx * pqx + pqy * pqy + pqz * pqz);
-      const double pref = kTwoPi52 * P.K * Q.K * rs;
-      double u[NR], w[NR];
-      rys_rule<NR>(X, xswitch, u, w, tab, istride);
-      const T tp = (T)P.p, tq = (T)Q.p, tipq = (T)ipq;
-      const T hp = (T)0.5 * nr_rcp(tp), hq = (T)0.5 * nr_rcp(tq);
-      const T tpqx = (T)pqx, tpqy = (T)pqy, tpqz = (T)pqz;
-#pragma unroll
-      for (int r = 0; r < NR; ++r) {
-        const T t2 = (T)u[r];
-        const T b00 = (T)0.5 * t2 * tipq;
-        const T b10 = hp - hp * tq * t2 * tipq;
-        const T b01 = hq - hq * tp * t2 * tipq;
-        const T fa = tq * tipq * t2, fc = tp * tipq * t2;
-        T gx[NI][NK], gy[NI][NK], gz[NI][NK];
-        rys_2d<NI, NK>(gx, (T)P.PAx - fa * tpqx, (T)Q.PAx + fc * tpqx, b10, b01, b00, (T)1);
-        rys_2d<NI, NK>(gy, (T)P.PAy - fa * tpqy, (T)Q.PAy + fc * tpqy, b10, b01, b00, (T)1);
-        rys_2d<NI, NK>(gz, (T)P.PAz - fa * tpqz, (T)Q.PAz + fc * tpqz, b10, b01, b00, (T)(w[r] * pref));
-#pragma unroll
-        for (int e = 0; e < NE; ++e)
-#pragma unroll
-          for (int f = 0; f < NF; ++f)
-            acc[e * NF + f] += gx[re.lx[e]][rf.lx[f]] * gy[re.ly[e]][rf.ly[f]] * gz[re.lz[e]][rf.lz[f]];
+      const PrimRec Q = prims[ket.prim_off + iq];
+      for (int ip = 0; ip < bra.nprim; ++ip) prim(prims[bra.prim_off + ip], Q);
+    }
+  } else {
+    for (int ip = 0; ip < bra.nprim; ++ip) {
+      const PrimRec P = prims[bra.prim_off + ip];
+      // (ss|ss): the next ket primitive is loaded one iteration ahead, its
+      // latency hidden behind this iteration's arithmetic (in the other
+      // classes the extra registers cost more occupancy than the latency it
+      // hides)
+      constexpr bool ahead = LA + LB + LC + LD == 0;
+      PrimRec Qn = prims[ket.prim_off];
+      for (int iq = 0; iq < ket.nprim; ++iq) {
+        const PrimRec Q = ahead ? Qn : prims[ket.prim_off + iq];
+        if (ahead && iq + 1 < ket.nprim) Qn = prims[ket.prim_off + iq + 1];
+        prim(P, Q);
       }
     }
   }
@@ -714,10 +728,10 @@ __global__ __launch_bounds__(256) void eri_row_kernel(ClassDev cd, BuildArgs ba)
     double v[S::NABCD];
     if (mixed_wave(ba, bra, ket)) {
       ++nlow;
-      small_integrals<float, LA, LB, LC, LD>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, v, tab,
+      small_integrals<float, LA, LB, LC, LD, !TR>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, v, tab,
                                              rys_lds_stride(S::NR));
     } else {
-      small_integrals<double, LA, LB, LC, LD>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, v, tab,
+      small_integrals<double, LA, LB, LC, LD, !TR>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, v, tab,
                                               rys_lds_stride(S::NR));
     }
 
