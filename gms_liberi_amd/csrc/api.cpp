@@ -82,6 +82,7 @@ struct ClassState {
   bool active = false;
   bool row_ok = false;  // a row-tiled kernel exists and fits in LDS
   int variant = -1;     // digestion chosen for the class: 0 direct, 1 row-tiled (-1: not yet)
+  float ms = -1.f;      // device time of the class's launch, measured at the first build
   int64_t nquart = 0, nprimq = 0;
   int32_t nseg = 0;
   DevBuf ket_list, seg_start, seg_bra, seg_ket0, tile_q, tile_seg;
@@ -300,8 +301,9 @@ int liberi_c_set_option(liberi_ctx *h, int32_t opt, double v) {
     case LIBERI_OPT_GRAPH: h->use_graph = v != 0.0; break;
     case LIBERI_OPT_AUTOTUNE:
       h->autotune = v != 0.0;
-      for (auto &c : h->cls) c.variant = -1;  // decide again at the next build,
-      set_tuned_choice(-1, -1);                // without earlier handles' choices
+      // decide (and time) again at the next build, without earlier handles' choices
+      for (auto &c : h->cls) c.variant = -1, c.ms = -1.f;
+      set_tuned_choice(-1, -1);
       break;
     default: return fail(LIBERI_EARG, "liberi_c_set_option: unknown option " + std::to_string(opt));
   }
@@ -828,7 +830,29 @@ static int choose_variants(liberi_ctx *h, const BuildArgs &ba, hipStream_t st) {
       best[v] = std::min(best[v], ms);
     }
     cs_.variant = best[1] < best[0] ? 1 : 0;
+    cs_.ms = std::min(best[0], best[1]);
     set_tuned_choice(c, cs_.variant);
+  }
+  // one timed launch of every class not timed above: the measured times
+  // order the classes over the side streams
+  for (int c = 0; c < kNumClasses; ++c) {
+    ClassState &cs_ = h->cls[c];
+    if (!cs_.active || cs_.ms >= 0.f || (h->only_class >= 0 && c != h->only_class)) continue;
+    if (!h->tune_F.p)
+      if (int rc = dev_alloc(h, h->tune_F, ntri * sizeof(double))) return rc;
+    for (auto &e : h->tev)
+      if (!e) HIP_TRY(hipEventCreate(&e));
+    BuildArgs bt = ba;
+    bt.F = (double *)h->tune_F.p;
+    ClassDev cd = cs_.dev;
+    cd.row_mode = cs_.row_ok && cd.t_end > cd.t_begin &&
+                  (h->row_tiles == 2 || (h->row_tiles == 1 && cs_.variant == 1));
+    HIP_TRY(hipEventRecord(h->tev[0], st));
+    if (int e = launch_class(c, cd, bt, st, kCoopWorkgroups))
+      return fail(LIBERI_EDEVICE, std::string("timing ") + kClasses[c].name + ": " + hipGetErrorString((hipError_t)e));
+    HIP_TRY(hipEventRecord(h->tev[1], st));
+    HIP_TRY(hipEventSynchronize(h->tev[1]));
+    HIP_TRY(hipEventElapsedTime(&cs_.ms, h->tev[0], h->tev[1]));
   }
   return LIBERI_OK;
 }
@@ -850,12 +874,14 @@ static int enqueue_build(liberi_ctx *h, const BuildArgs &ba, const double *dD, d
   for (int c = 0; c < kNumClasses; ++c)
     if (h->cls[c].active && (h->only_class < 0 || c == h->only_class)) order.push_back(c);
   std::vector<double> load(ns, 0.0);
-  // cost proxy: primitive quartets x Rys roots x sqrt(cartesian components of the class)
+  // cost: the class's measured time (first build), else a proxy scaled to
+  // milliseconds: primitive quartets x Rys roots x sqrt(cartesian components)
   auto cost = [&](int c) {
+    if (h->cls[c].ms >= 0.f) return (double)h->cls[c].ms;
     const int la = kPairL[kClasses[c].bra][1], lb = kPairL[kClasses[c].bra][0];
     const int lc = kPairL[kClasses[c].ket][1], ld = kPairL[kClasses[c].ket][0];
     const double ncomp = (double)ncart_host(la) * ncart_host(lb) * ncart_host(lc) * ncart_host(ld);
-    return (double)h->cls[c].nprimq * class_nroots(c) * std::sqrt(ncomp);
+    return 1e-9 * (double)h->cls[c].nprimq * class_nroots(c) * std::sqrt(ncomp);
   };
   if (ns > 1) {
     std::sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
