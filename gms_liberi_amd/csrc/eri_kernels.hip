@@ -1955,9 +1955,13 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
   // NQ): consecutive rounds stay in the same or the next segment, so each
   // lane searches the segments once and then steps forward, and they share
   // the bra pair (its records and density blocks stay in cache)
+  // Workgroups resident at the same time take chunks spread over the whole
+  // quartet range (chunk = workgroup x 40009 mod grid, a permutation for any
+  // grid below that prime), so their atomics meet fewer common Fock rows.
   const int64_t nq = cd.q_end - cd.q_begin;
   const int64_t per = ((nq + gridDim.x - 1) / gridDim.x + NQ - 1) / NQ * NQ;
-  const int64_t c0 = (int64_t)blockIdx.x * per, c1 = c0 + per < nq ? c0 + per : nq;
+  const int64_t chunk = (int64_t)blockIdx.x * 40009 % gridDim.x;
+  const int64_t c0 = chunk * per, c1 = c0 + per < nq ? c0 + per : nq;
   int seg = -1;
   for (int64_t base = c0; base < c1; base += NQ) {
     const bool valid = !idle && base + slot < c1;
