@@ -136,7 +136,7 @@ constexpr int64_t kRowLdsBytes = 64 * 1024;
 // workgroups the cooperative kernels use
 constexpr int kCoopWorkgroups = 2048;
 // one-wave workgroups the group-per-quartet kernels use at most (persistent loop)
-constexpr int kGroupWorkgroups = 32768;
+constexpr int kGroupWorkgroups = 65536;
 
 // Schwarz values sqrt(max |(ab|ab)|) of the n shell pairs list[0..n) whose
 // shells have angular momenta la >= lb (A the la shell); out[i] for list[i].
