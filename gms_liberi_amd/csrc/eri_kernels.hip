@@ -1540,14 +1540,17 @@ __device__ __forceinline__ int gpos_of(int L, int k) {
   return gamess_pos(x, y, z);
 }
 
-// Persistent loop over the class's quartets, NQ per one-wave workgroup and
-// round, digestion by atomic adds to F.  (Keeping exchange blocks on chip --
-// per row tile in LDS rows, or per run of kets sharing a shell -- cut this
-// kernel's atomics 5-9x on C60 but not its time: its atomics overlap its
-// arithmetic, and the LDS rows cost occupancy.)
-// (classes of s, p and d shells up to total angular momentum 7 fit 168
-// VGPRs, i.e. 3 waves per SIMD, with at most 12 bytes of spill per lane in
-// (dp|dp) and (dd|dp); measured faster than 2 waves without spill)
+// Persistent loop over a contiguous chunk of the class's quartets, NQ per
+// one-wave workgroup and round: rule records, 2-D tables and accumulation
+// per batch of primitive quartets, the HRR, then digestion by atomic adds to
+// F (the bra pair's Coulomb block summed in registers while consecutive
+// rounds keep the bra pair).  Keeping the exchange blocks on chip -- per row
+// tile in LDS rows, or per run of kets sharing a shell -- cut this kernel's
+// atomics 5-9x on C60 but not its time: its atomics overlap its arithmetic,
+// and the LDS rows cost occupancy.  Classes of s, p and d shells up to total
+// angular momentum 7 fit 168 VGPRs, i.e. 3 waves per SIMD, with at most 20
+// bytes of spill per lane ((dp|dp), (dd|ds), (dd|dp)); measured faster than
+// 2 waves without spill.
 template <int LA, int LB, int LC, int LD>
 __attribute__((amdgpu_waves_per_eu(LA <= 2 && LB <= 2 && LC <= 2 && LD <= 2 && LA + LB + LC + LD <= 7 ? 3 : 1, 8)))
 __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba) {
@@ -1586,7 +1589,8 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
     } else {
       offx[k] = offy[k] = offz[k] = 0;
     }
-  }  // MAP 1/2: the lane's row e (column f) and root part
+  }
+  // MAP 1/2: the lane's row e (column f) and root part
   constexpr int P = S::P, NRP = P ? (NR + P - 1) / P : 1;
   const int part = P ? gl % P : 0;
   int rx = 0, ry = 0, rz = 0;
@@ -1596,7 +1600,6 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
   }
   // table element of the lane's row (MAP 1: [rx][j]) or column (MAP 2: [j][rx])
   const int rowx = S::MAP == 1 ? rx * NK : rx, rowy = S::MAP == 1 ? ry * NK : ry, rowz = S::MAP == 1 ? rz * NK : rz;
-
 
   // the slot's J(AB) block for the run of quartets sharing bra pair jab_pair
   constexpr int NJL = (S::NAB + G - 1) / G;
