@@ -792,8 +792,35 @@ static void set_tuned_choice(int c, int v) {
   }
 }
 
-static int choose_variants(liberi_ctx *h, const BuildArgs &ba, hipStream_t st) {
+// Whether class c runs its row-tiled kernel in this build.
+static bool class_row_mode(const liberi_ctx *h, int c) {
+  const ClassState &cs_ = h->cls[c];
+  return cs_.row_ok && cs_.dev.t_end > cs_.dev.t_begin &&
+         (h->row_tiles == 2 || (h->row_tiles == 1 && cs_.variant == 1));
+}
+
+// One launch of class c (row-tiled or direct) into the scratch Fock matrix,
+// timed with events on st.
+static int time_class(liberi_ctx *h, const BuildArgs &ba, hipStream_t st, int c, bool row_mode, float &ms) {
   const int64_t ntri = (int64_t)h->num_bas * (h->num_bas + 1) / 2;
+  if (!h->tune_F.p)
+    if (int rc = dev_alloc(h, h->tune_F, ntri * sizeof(double))) return rc;
+  for (auto &e : h->tev)
+    if (!e) HIP_TRY(hipEventCreate(&e));
+  BuildArgs bt = ba;
+  bt.F = (double *)h->tune_F.p;
+  ClassDev cd = h->cls[c].dev;
+  cd.row_mode = row_mode;
+  HIP_TRY(hipEventRecord(h->tev[0], st));
+  if (int e = launch_class(c, cd, bt, st, kCoopWorkgroups))
+    return fail(LIBERI_EDEVICE, std::string("timing ") + kClasses[c].name + ": " + hipGetErrorString((hipError_t)e));
+  HIP_TRY(hipEventRecord(h->tev[1], st));
+  HIP_TRY(hipEventSynchronize(h->tev[1]));
+  HIP_TRY(hipEventElapsedTime(&ms, h->tev[0], h->tev[1]));
+  return LIBERI_OK;
+}
+
+static int choose_variants(liberi_ctx *h, const BuildArgs &ba, hipStream_t st) {
   for (int c = 0; c < kNumClasses; ++c) {
     ClassState &cs_ = h->cls[c];
     if (!cs_.active || cs_.variant >= 0) continue;
@@ -808,25 +835,11 @@ static int choose_variants(liberi_ctx *h, const BuildArgs &ba, hipStream_t st) {
       cs_.variant = prev;
       continue;
     }
-    if (!h->tune_F.p)
-      if (int rc = dev_alloc(h, h->tune_F, ntri * sizeof(double))) return rc;
-    for (auto &e : h->tev)
-      if (!e) HIP_TRY(hipEventCreate(&e));
-    BuildArgs bt = ba;
-    bt.F = (double *)h->tune_F.p;
     float best[2] = {1e30f, 1e30f};
     for (int rep = 0; rep < 4; ++rep) {
       const int v = rep & 1;
-      ClassDev cd = cs_.dev;
-      cd.row_mode = v;
-      HIP_TRY(hipEventRecord(h->tev[0], st));
-      int e = launch_class(c, cd, bt, st, kCoopWorkgroups);
-      if (e != hipSuccess)
-        return fail(LIBERI_EDEVICE, std::string("autotune ") + kClasses[c].name + ": " + hipGetErrorString((hipError_t)e));
-      HIP_TRY(hipEventRecord(h->tev[1], st));
-      HIP_TRY(hipEventSynchronize(h->tev[1]));
       float ms = 0.f;
-      HIP_TRY(hipEventElapsedTime(&ms, h->tev[0], h->tev[1]));
+      if (int rc = time_class(h, ba, st, c, v == 1, ms)) return rc;
       best[v] = std::min(best[v], ms);
     }
     cs_.variant = best[1] < best[0] ? 1 : 0;
@@ -838,21 +851,7 @@ static int choose_variants(liberi_ctx *h, const BuildArgs &ba, hipStream_t st) {
   for (int c = 0; c < kNumClasses; ++c) {
     ClassState &cs_ = h->cls[c];
     if (!cs_.active || cs_.ms >= 0.f || (h->only_class >= 0 && c != h->only_class)) continue;
-    if (!h->tune_F.p)
-      if (int rc = dev_alloc(h, h->tune_F, ntri * sizeof(double))) return rc;
-    for (auto &e : h->tev)
-      if (!e) HIP_TRY(hipEventCreate(&e));
-    BuildArgs bt = ba;
-    bt.F = (double *)h->tune_F.p;
-    ClassDev cd = cs_.dev;
-    cd.row_mode = cs_.row_ok && cd.t_end > cd.t_begin &&
-                  (h->row_tiles == 2 || (h->row_tiles == 1 && cs_.variant == 1));
-    HIP_TRY(hipEventRecord(h->tev[0], st));
-    if (int e = launch_class(c, cd, bt, st, kCoopWorkgroups))
-      return fail(LIBERI_EDEVICE, std::string("timing ") + kClasses[c].name + ": " + hipGetErrorString((hipError_t)e));
-    HIP_TRY(hipEventRecord(h->tev[1], st));
-    HIP_TRY(hipEventSynchronize(h->tev[1]));
-    HIP_TRY(hipEventElapsedTime(&cs_.ms, h->tev[0], h->tev[1]));
+    if (int rc = time_class(h, ba, st, c, class_row_mode(h, c), cs_.ms)) return rc;
   }
   return LIBERI_OK;
 }
@@ -903,8 +902,7 @@ static int enqueue_build(liberi_ctx *h, const BuildArgs &ba, const double *dD, d
       HIP_TRY(hipEventRecord(h->cev0[c], cs));
     }
     ClassDev cd = h->cls[c].dev;
-    cd.row_mode = h->cls[c].row_ok && cd.t_end > cd.t_begin &&
-                  (h->row_tiles == 2 || (h->row_tiles == 1 && h->cls[c].variant == 1));
+    cd.row_mode = class_row_mode(h, c);
     int e = launch_class(c, cd, ba, cs, kCoopWorkgroups);
     if (e != hipSuccess)
       return fail(LIBERI_EDEVICE, std::string("launch ") + kClasses[c].name + ": " + hipGetErrorString((hipError_t)e));
