@@ -23,7 +23,7 @@ REF = Path(os.environ.get("LIBERI_REFERENCE", "/root/reference"))
 OUT = Path(__file__).resolve().parent / "ref_screening_w2_631gd.json"
 
 # class labels of the target array (comment at tests/test_screening.py:379)
-# mapped to this build's kernel names (dispatch names of src/liberi_driver.F90)
+# mapped to the class names used by the library (dispatch names of src/liberi_driver.F90)
 LABELS = {"0001": "int0001", "0011": "int0011", "0111": "int0111", "0002": "int0002", "0012": "int0012",
           "0112": "int0112", "1112": "int2111", "0022": "int0022", "0122": "int0122", "1122": "int2211",
           "0222": "int2220", "1222": "int2221", "0102": "int0102", "0212": "int2120"}

@@ -1,6 +1,6 @@
 """HIP Fock build against the reference's outputs and the oracle (GPU).
 
-Bar (north star): max-abs Fock difference <= 1e-10 and |dE_2e| <= 1e-9 Eh
+Bar: max-abs Fock difference <= 1e-10 and |dE_2e| <= 1e-9 Eh
 against the reference's own output on the same inputs (tests/golden, written
 by make_reference_fixtures.py), surviving shell-quartet counts bit-exact.
 
