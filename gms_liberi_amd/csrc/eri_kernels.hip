@@ -1387,6 +1387,18 @@ constexpr int lds_cycles(const int *a, int n) {
   return m;
 }
 
+// Slot of a group (unpadded, doubles): the union of table and HRR buffers
+// (uni), the density blocks when they do not fit the free HRR buffer (dbx),
+// and room for as many rule records of the primitive loop as one pass of the
+// wave makes, but never more than the slot with separate density blocks
+// (db) holds
+__host__ __device__ constexpr int grp_slot(int uni, int gs, int kt, int nq, int nr, int db, int dbx) {
+  const int rtw = 64 / (nq * nr) / kt * kt, need = kt * gs + rtw * (2 * nr + 14);
+  const int lo = uni + dbx, hi = uni + db;
+  const int want = need < hi ? need : hi;
+  return want > lo ? want : lo;
+}
+
 // Padding of the 2-D table stride (gsz0 + gp doubles) and of the slot
 // (+ sp doubles), chosen to minimise the LDS cycles of the primitive loop:
 // the accumulation reads (one instruction reads the same table element
@@ -1398,7 +1410,7 @@ constexpr int lds_cycles(const int *a, int n) {
 // (0 for the lane-per-quartet classes, which have no slots, and for strided
 // maps of more than 128 components; strided maps pad the slot only).
 template <int LA, int LB, int LC, int LD>
-constexpr int grp_pad(int map, int p, int nq, int g, int nr, int gsz0, int bufp, int db) {
+constexpr int grp_pad(int map, int p, int nq, int g, int nr, int gsz0, int bufp, int db, int dbx) {
   constexpr int LAB = LA + LB, LCD = LC + LD, NI = LAB + 1, NK = LCD + 1;
   constexpr Range<LA, LAB> re{};
   constexpr Range<LC, LCD> rf{};
@@ -1410,9 +1422,10 @@ constexpr int grp_pad(int map, int p, int nq, int g, int nr, int gsz0, int bufp,
   for (int gp = 0; gp < (map == 0 ? 1 : 16); ++gp)
     for (int sp = 0; sp < 16; ++sp) {
       const int gsz = gsz0 + gp, gs = 3 * nr * gsz;
-      const int uni = 2 * gs > 2 * bufp ? 2 * gs : 2 * bufp, slot = uni + db + sp;
+      const int uni = 2 * gs > 2 * bufp ? 2 * gs : 2 * bufp;
       int kt = g / (3 * nr) < uni / gs ? g / (3 * nr) : uni / gs;
       kt = kt < 1 ? 1 : kt;
+      const int slot = grp_slot(uni, gs, kt, nq, nr, db, dbx) + sp;
       if (gp == 0 && sp == 0) {
         kt0 = kt;
         fit3 = nq * slot * 8 <= kGroupLdsTarget;
@@ -1479,10 +1492,13 @@ struct Grp {
   // buffers of the HRR: bra phase [e][b][f] (stride NF), ket phase [ab][KS]
   static constexpr int BUFP = grp_buffer(LA, LB, LC, LD, KS);
   // slot before padding: table buffers (at least two), dead once the
-  // primitive loop ends, aliased by the HRR buffers; the density blocks
+  // primitive loop ends, aliased by the HRR buffers; the density blocks of
+  // the digestion go into the HRR buffer the last level left free when they
+  // fit there (DBX = 0), else after the buffers (DBX doubles)
   static constexpr int GSZ0 = NI * NK;
   static constexpr int UNI0 = 2 * 3 * NR * GSZ0 > 2 * BUFP ? 2 * 3 * NR * GSZ0 : 2 * BUFP;
-  static constexpr int SLOT0 = UNI0 + DB;
+  static constexpr int DBX = DB <= BUFP ? 0 : DB;
+  static constexpr int SLOT0 = UNI0 + DBX;
   // lane map of the primitive loop.  MAP 1: a lane per row e of (e0|f0)
   // accumulating all its columns f; MAP 2: a lane per column f accumulating
   // all rows e; in both the roots are split over P lanes (partial sums added
@@ -1502,16 +1518,16 @@ struct Grp {
   static constexpr int NKC = (NE * NF + G - 1) / G;
   // table stride (one 2-D table of NI x NK, padded) and slot, padded
   // against LDS bank conflicts
-  static constexpr int PAD = grp_pad<LA, LB, LC, LD>(MAP, P, NQ, G, NR, GSZ0, BUFP, DB);
+  static constexpr int PAD = grp_pad<LA, LB, LC, LD>(MAP, P, NQ, G, NR, GSZ0, BUFP, DB, DBX);
   static constexpr int GSZ = GSZ0 + PAD / 16;
   static constexpr int GS = 3 * NR * GSZ;
   static constexpr int UNI = 2 * GS > 2 * BUFP ? 2 * GS : 2 * BUFP;
-  static constexpr int SLOT = UNI + DB + PAD % 16;
-  static constexpr bool fits = SLOT * NQ * 8 <= 64 * 1024;
   // primitive quartets whose 2-D tables one pass of the table lanes makes
   // (3 NR lanes each), held in the HRR space of the slot
   static constexpr int KT0 = G / (3 * NR) < UNI / GS ? G / (3 * NR) : UNI / GS;
   static constexpr int KT = KT0 < 1 ? 1 : KT0;
+  static constexpr int SLOT = grp_slot(UNI, GS, KT, NQ, NR, DB, DBX) + PAD % 16;
+  static constexpr bool fits = SLOT * NQ * 8 <= 64 * 1024;
   // rule records of the primitive loop, one per primitive quartet:
   // u[NR], w[NR] (times 2 pi^5/2 K_P K_Q / sqrt(p+q)), 1/(p+q), q/(p+q),
   // p/(p+q), 1/2p, 1/2q, P-Q, P-A, Q-C.  They live in the part of the slot
@@ -1570,9 +1586,7 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
   const int slot = idle ? NQ - 1 : lane / G;
   const int gl = idle ? (1 << 20) : lane - slot * G;
   double *Gt = lds + slot * S::SLOT;
-  double *buf0 = Gt, *buf1 = buf0 + S::BUFP, *Db = Gt + S::UNI;
-  double *DAB = Db, *DCD = DAB + S::NAB, *DAC = DCD + S::NCD, *DBD = DAC + S::NAC, *DAD = DBD + S::NBD,
-         *DBC = DAD + S::NAD;
+  double *buf0 = Gt, *buf1 = buf0 + S::BUFP;
 
   // MAP 0: table offsets of the lane's components
   int offx[NKC], offy[NKC], offz[NKC];
@@ -1845,6 +1859,10 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
       double *t = in; in = out; out = t;
     }
     constexpr int KS = S::KS, NCD = NC * ND;
+    // the six density blocks: in the free HRR buffer (at least BUFP doubles
+    // from `out`) or after the buffers
+    double *DAB = S::DBX ? Gt + S::UNI : out, *DCD = DAB + S::NAB, *DAC = DCD + S::NCD, *DBD = DAC + S::NAC,
+           *DAD = DBD + S::NBD, *DBC = DAD + S::NAD;
     // scale + value cutoff in place (rows ab of stride KS); gather the six density blocks
     if (valid) {
       constexpr Range<LC, LC> rc{};
