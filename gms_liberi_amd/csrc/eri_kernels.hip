@@ -1550,6 +1550,24 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Lanes per output of a digestion block of `no` outputs on a slot of g
+// lanes, splitting a summed index of extent x: the largest power of two
+// that keeps no x split <= g and the split <= x
+__host__ __device__ constexpr int split_lanes(int no, int x, int g) {
+  int s = 1;
+  while (2 * s * no <= g && 2 * s <= x) s *= 2;
+  return s;
+}
+
+// Sum of the partial sums of SP adjacent lanes (SP a power of two) into the
+// first of them; the other lanes' results are meaningless
+template <int SP>
+__device__ __forceinline__ double part_sum(double t) {
+#pragma unroll
+  for (int d = SP / 2; d >= 1; d /= 2) t += __shfl_down(t, d, 64);
+  return t;
+}
+
 __device__ __forceinline__ int gpos_of(int L, int k) {
   int x, y, z;
   cart_decode(L, k, x, y, z);
@@ -1916,57 +1934,88 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
           jab[k] += 4.0 * s * t;
         }
       }
-      // J(CD): F[cd] += 4s sum_ab v D[ab]
-      for (int i = gl; i < NCD; i += G) {
-        const int c = i / ND, d = i - c * ND;
-        double t = 0.0;
+      // The other five blocks: an output's sum runs over two component
+      // indices; when the slot's lanes outnumber the outputs, the outer one is
+      // split over SP adjacent lanes (part p takes p, p + SP, ...) whose
+      // partial sums meet in part 0.  Offsets stay compile-time past a
+      // per-lane base.
+      // J(CD): F[cd] += 4s sum_ab v D[ab], rows ab split
+      {
+        constexpr int SP = split_lanes(NCD, S::NAB, G);
+        for (int i0 = gl; i0 < NCD * SP; i0 += G) {
+          const int i = i0 / SP, p = i0 - i * SP, c = i / ND, d = i - c * ND;
+          const double *vb = v + p * KS + i, *db = DAB + p;
+          double t = 0.0;
 #pragma unroll
-        for (int e = 0; e < S::NAB; ++e) t += v[e * KS + i] * DAB[e];
-        put(ket.bfA + gpos_rt(LC, c), ket.bfB + gpos_rt(LD, d), 4.0 * s * t);
+          for (int k = 0; k * SP < S::NAB; ++k)
+            if (S::NAB % SP == 0 || p + k * SP < S::NAB) t += vb[k * SP * KS] * db[k * SP];
+          t = part_sum<SP>(t);
+          if (p == 0) put(ket.bfA + gpos_rt(LC, c), ket.bfB + gpos_rt(LD, d), 4.0 * s * t);
+        }
       }
-      // K(AC): F[ac] -= s sum_bd v D[bd]
-      for (int i = gl; i < S::NAC; i += G) {
-        const int a = i / NC, c = i - a * NC;
-        double t = 0.0;
+      // K(AC): F[ac] -= s sum_bd v D[bd], b split
+      {
+        constexpr int SP = split_lanes(S::NAC, NB, G);
+        for (int i0 = gl; i0 < S::NAC * SP; i0 += G) {
+          const int i = i0 / SP, p = i0 - i * SP, a = i / NC, c = i - a * NC;
+          const double *vb = v + (a * NB + p) * KS + c * ND, *db = DBD + p * ND;
+          double t = 0.0;
 #pragma unroll
-        for (int b = 0; b < NB; ++b)
+          for (int k = 0; k * SP < NB; ++k)
+            if (NB % SP == 0 || p + k * SP < NB)
 #pragma unroll
-          for (int d = 0; d < ND; ++d) t += v[(a * NB + b) * KS + c * ND + d] * DBD[b * ND + d];
-        const int pa = gpos_rt(LA, a), pc = gpos_rt(LC, c);
-        put(bra.bfA + pa, ket.bfA + pc, -s * ba.k_scale * t);
+              for (int d = 0; d < ND; ++d) t += vb[k * SP * KS + d] * db[k * SP * ND + d];
+          t = part_sum<SP>(t);
+          if (p == 0) put(bra.bfA + gpos_rt(LA, a), ket.bfA + gpos_rt(LC, c), -s * ba.k_scale * t);
+        }
       }
-      // K(BD): F[bd] -= s sum_ac v D[ac]
-      for (int i = gl; i < S::NBD; i += G) {
-        const int b = i / ND, d = i - b * ND;
-        double t = 0.0;
+      // K(BD): F[bd] -= s sum_ac v D[ac], a split
+      {
+        constexpr int SP = split_lanes(S::NBD, NA, G);
+        for (int i0 = gl; i0 < S::NBD * SP; i0 += G) {
+          const int i = i0 / SP, p = i0 - i * SP, b = i / ND, d = i - b * ND;
+          const double *vb = v + (p * NB + b) * KS + d, *db = DAC + p * NC;
+          double t = 0.0;
 #pragma unroll
-        for (int a = 0; a < NA; ++a)
+          for (int k = 0; k * SP < NA; ++k)
+            if (NA % SP == 0 || p + k * SP < NA)
 #pragma unroll
-          for (int c = 0; c < NC; ++c) t += v[(a * NB + b) * KS + c * ND + d] * DAC[a * NC + c];
-        const int pb = gpos_rt(LB, b), pd = gpos_rt(LD, d);
-        put(bra.bfB + pb, ket.bfB + pd, -s * ba.k_scale * t);
+              for (int c = 0; c < NC; ++c) t += vb[k * SP * NB * KS + c * ND] * db[k * SP * NC + c];
+          t = part_sum<SP>(t);
+          if (p == 0) put(bra.bfB + gpos_rt(LB, b), ket.bfB + gpos_rt(LD, d), -s * ba.k_scale * t);
+        }
       }
-      // K(AD): F[ad] -= s sum_bc v D[bc]
-      for (int i = gl; i < S::NAD; i += G) {
-        const int a = i / ND, d = i - a * ND;
-        double t = 0.0;
+      // K(AD): F[ad] -= s sum_bc v D[bc], b split
+      {
+        constexpr int SP = split_lanes(S::NAD, NB, G);
+        for (int i0 = gl; i0 < S::NAD * SP; i0 += G) {
+          const int i = i0 / SP, p = i0 - i * SP, a = i / ND, d = i - a * ND;
+          const double *vb = v + (a * NB + p) * KS + d, *db = DBC + p * NC;
+          double t = 0.0;
 #pragma unroll
-        for (int b = 0; b < NB; ++b)
+          for (int k = 0; k * SP < NB; ++k)
+            if (NB % SP == 0 || p + k * SP < NB)
 #pragma unroll
-          for (int c = 0; c < NC; ++c) t += v[(a * NB + b) * KS + c * ND + d] * DBC[b * NC + c];
-        const int pa = gpos_rt(LA, a), pd = gpos_rt(LD, d);
-        put(bra.bfA + pa, ket.bfB + pd, -s * ba.k_scale * t);
+              for (int c = 0; c < NC; ++c) t += vb[k * SP * KS + c * ND] * db[k * SP * NC + c];
+          t = part_sum<SP>(t);
+          if (p == 0) put(bra.bfA + gpos_rt(LA, a), ket.bfB + gpos_rt(LD, d), -s * ba.k_scale * t);
+        }
       }
-      // K(BC): F[bc] -= s sum_ad v D[ad]
-      for (int i = gl; i < S::NBC; i += G) {
-        const int b = i / NC, c = i - b * NC;
-        double t = 0.0;
+      // K(BC): F[bc] -= s sum_ad v D[ad], a split
+      {
+        constexpr int SP = split_lanes(S::NBC, NA, G);
+        for (int i0 = gl; i0 < S::NBC * SP; i0 += G) {
+          const int i = i0 / SP, p = i0 - i * SP, b = i / NC, c = i - b * NC;
+          const double *vb = v + (p * NB + b) * KS + c * ND, *db = DAD + p * ND;
+          double t = 0.0;
 #pragma unroll
-        for (int a = 0; a < NA; ++a)
+          for (int k = 0; k * SP < NA; ++k)
+            if (NA % SP == 0 || p + k * SP < NA)
 #pragma unroll
-          for (int d = 0; d < ND; ++d) t += v[(a * NB + b) * KS + c * ND + d] * DAD[a * ND + d];
-        const int pb = gpos_rt(LB, b), pc = gpos_rt(LC, c);
-        put(bra.bfB + pb, ket.bfA + pc, -s * ba.k_scale * t);
+              for (int d = 0; d < ND; ++d) t += vb[k * SP * NB * KS + d] * db[k * SP * ND + d];
+          t = part_sum<SP>(t);
+          if (p == 0) put(bra.bfB + gpos_rt(LB, b), ket.bfA + gpos_rt(LC, c), -s * ba.k_scale * t);
+        }
       }
     }
     wave_sync();  // slots are reused by the next round
