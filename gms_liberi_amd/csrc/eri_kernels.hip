@@ -1578,12 +1578,13 @@ __device__ __forceinline__ int gpos_of(int L, int k) {
 // one-wave workgroup and round: rule records, 2-D tables and accumulation
 // per batch of primitive quartets, the HRR, then digestion by atomic adds to
 // F (the bra pair's Coulomb block summed in registers while consecutive
-// rounds keep the bra pair).  Keeping the exchange blocks on chip -- per row
+// rounds keep the bra pair; the sums of blocks with fewer outputs than lanes
+// split over the idle lanes).  Keeping the exchange blocks on chip -- per row
 // tile in LDS rows, or per run of kets sharing a shell -- cut this kernel's
 // atomics 5-9x on C60 but not its time: its atomics overlap its arithmetic,
 // and the LDS rows cost occupancy.  Classes of s, p and d shells up to total
-// angular momentum 7 fit 168 VGPRs, i.e. 3 waves per SIMD, with at most 20
-// bytes of spill per lane ((dp|dp), (dd|ds), (dd|dp)); measured faster than
+// angular momentum 7 fit 168 VGPRs, i.e. 3 waves per SIMD, with at most 40
+// bytes of spill per lane ((dp|dp), (dd|pp), (dd|dp)); measured faster than
 // 2 waves without spill.
 template <int LA, int LB, int LC, int LD>
 __attribute__((amdgpu_waves_per_eu(LA <= 2 && LB <= 2 && LC <= 2 && LD <= 2 && LA + LB + LC + LD <= 7 ? 3 : 1, 8)))
