@@ -1399,6 +1399,27 @@ __host__ __device__ constexpr int grp_slot(int uni, int gs, int kt, int nq, int 
   return want > lo ? want : lo;
 }
 
+// LDS cycles of one wave-wide 64-bit access, lane l at address a[l] (in
+// doubles; -1: lane inactive): the LDS serves 16 lanes of such an access per
+// clock, so each 16-lane group costs the largest number of distinct
+// addresses its lanes put on one bank pair (equal addresses broadcast)
+constexpr int lds_cycles_b64(const int *a, int n) {
+  int total = 0;
+  for (int g0 = 0; g0 < n; g0 += 16) {
+    int cnt[16] = {};
+    for (int i = g0; i < g0 + 16 && i < n; ++i) {
+      if (a[i] < 0) continue;
+      bool dup = false;
+      for (int j = g0; j < i; ++j) dup = dup || a[j] == a[i];
+      if (!dup) ++cnt[a[i] % 16];
+    }
+    int m = 0;
+    for (int k = 0; k < 16; ++k) m = cnt[k] > m ? cnt[k] : m;
+    total += m;
+  }
+  return total;
+}
+
 // Padding of the 2-D table stride (gsz0 + gp doubles) and of the slot
 // (+ sp doubles), chosen to minimise the LDS cycles of the primitive loop:
 // the accumulation reads (one instruction reads the same table element
@@ -1435,12 +1456,19 @@ constexpr int grp_pad(int map, int p, int nq, int g, int nr, int gsz0, int bufp,
       int cost = 0;
       // accumulation reads
       if (map == 1 || map == 2) {
-        const int nx = map == 1 ? NI : NK, xs = map == 1 ? NK : 1, nj = map == 1 ? NK : NI;
-        int n = 0;
-        for (int q = 0; q < nq; ++q)
-          for (int pp = 0; pp < p; ++pp)
-            for (int x = 0; x < nx; ++x) addr[n++] = q * slot + pp * gsz + x * xs;
-        cost += kt * 3 * nj * ((nr + p - 1) / p) * lds_cycles(addr, n);
+        // lane q g + row p + part reads row (column) `row` of its part's
+        // table: offset part gsz + (its exponent) xs
+        const int xs = map == 1 ? NK : 1, nj = map == 1 ? NK : NI;
+        for (int dim = 0; dim < 3; ++dim) {
+          for (int l = 0; l < 64; ++l) {
+            const int q = l / g, gl = l - q * g, row = gl / p, pp = gl - row * p;
+            int x = 0;
+            if (map == 1 && row < NE) x = dim == 0 ? re.lx[row] : (dim == 1 ? re.ly[row] : re.lz[row]);
+            if (map == 2 && row < NF) x = dim == 0 ? rf.lx[row] : (dim == 1 ? rf.ly[row] : rf.lz[row]);
+            addr[l] = q < nq ? q * slot + (dim * nr + pp) * gsz + x * xs : -1;
+          }
+          cost += kt * nj * ((nr + p - 1) / p) * lds_cycles_b64(addr, 64);
+        }
       } else {
         for (int k = 0; k * g < NE * NF; ++k)
           for (int dim = 0; dim < 3; ++dim) {
@@ -1460,8 +1488,14 @@ constexpr int grp_pad(int map, int p, int nq, int g, int nr, int gsz0, int bufp,
             cost += kt * nr * lds_cycles(addr, n);
           }
       }
-      // table writes
-      {
+      // table writes: lane q g + tt 3 nr + task writes table (task % 3, task / 3) of buffer tt
+      if (map == 1 || map == 2) {
+        for (int l = 0; l < 64; ++l) {
+          const int q = l / g, gl = l - q * g, tt = gl / (3 * nr), task = gl - tt * (3 * nr);
+          addr[l] = q < nq && tt < kt ? q * slot + tt * gs + ((task % 3) * nr + task / 3) * gsz : -1;
+        }
+        cost += NI * NK * lds_cycles_b64(addr, 64);
+      } else {
         int n = 0;
         for (int q = 0; q < nq; ++q)
           for (int m = 0; m < kt * 3 * nr && n < 64; ++m) addr[n++] = q * slot + m * gsz;
