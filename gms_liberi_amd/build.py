@@ -22,11 +22,8 @@ LIB = PKG / "libliberi_hip.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-# -fconstexpr-steps: the compile-time LDS padding search of the group kernels
-# (grp_pad) evaluates every candidate's bank conflicts lane by lane
 COMMON = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
-          "-fconstexpr-steps=67108864", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-I", str(PKG.parent / "include")]
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", str(PKG.parent / "include")]
 
 # class-kernel translation units: LIBERI_CLASS_GROUP selects which classes a unit instantiates
 KERNEL_GROUPS = 8

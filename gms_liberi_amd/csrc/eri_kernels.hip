@@ -1400,19 +1400,15 @@ __host__ __device__ constexpr int grp_slot(int uni, int gs, int kt, int nq, int 
 }
 
 // LDS cycles of one wave-wide 64-bit access, lane l at address a[l] (in
-// doubles; -1: lane inactive): the LDS serves 16 lanes of such an access per
-// clock, so each 16-lane group costs the largest number of distinct
-// addresses its lanes put on one bank pair (equal addresses broadcast)
+// doubles, all distinct; -1: lane inactive): the LDS serves 16 lanes of such
+// an access per clock, so each 16-lane group costs the largest number of
+// its addresses on one bank pair
 constexpr int lds_cycles_b64(const int *a, int n) {
   int total = 0;
   for (int g0 = 0; g0 < n; g0 += 16) {
     int cnt[16] = {};
-    for (int i = g0; i < g0 + 16 && i < n; ++i) {
-      if (a[i] < 0) continue;
-      bool dup = false;
-      for (int j = g0; j < i; ++j) dup = dup || a[j] == a[i];
-      if (!dup) ++cnt[a[i] % 16];
-    }
+    for (int i = g0; i < g0 + 16 && i < n; ++i)
+      if (a[i] >= 0) ++cnt[a[i] % 16];
     int m = 0;
     for (int k = 0; k < 16; ++k) m = cnt[k] > m ? cnt[k] : m;
     total += m;
@@ -1440,6 +1436,28 @@ constexpr int grp_pad(int map, int p, int nq, int g, int nr, int gsz0, int bufp,
   if (map == 0 && NE * NF > 128) return 0;  // (keeps the search within the compiler's step budget)
   int best = 0, best_cost = -1, kt0 = 0;
   bool fit3 = false;
+  // row/column maps: the distinct (slot, root part, exponent) triples each
+  // 16-lane group of an accumulation read touches, per direction (lanes
+  // with equal triples read the same address); independent of the padding
+  int tq[3][4][16] = {}, tp[3][4][16] = {}, tx[3][4][16] = {}, tn[3][4] = {};
+  if (map == 1 || map == 2)
+    for (int dim = 0; dim < 3; ++dim)
+      for (int l = 0; l < 64; ++l) {
+        const int q = l / g, gl = l - q * g, row = gl / p, pp = gl - row * p, grp = l / 16;
+        if (q >= nq) continue;
+        int x = 0;
+        if (map == 1 && row < NE) x = dim == 0 ? re.lx[row] : (dim == 1 ? re.ly[row] : re.lz[row]);
+        if (map == 2 && row < NF) x = dim == 0 ? rf.lx[row] : (dim == 1 ? rf.ly[row] : rf.lz[row]);
+        bool dup = false;
+        for (int j = 0; j < tn[dim][grp]; ++j)
+          dup = dup || (tq[dim][grp][j] == q && tp[dim][grp][j] == pp && tx[dim][grp][j] == x);
+        if (!dup) {
+          const int j = tn[dim][grp]++;
+          tq[dim][grp][j] = q;
+          tp[dim][grp][j] = pp;
+          tx[dim][grp][j] = x;
+        }
+      }
   for (int gp = 0; gp < (map == 0 ? 1 : 16); ++gp)
     for (int sp = 0; sp < 16; ++sp) {
       const int gsz = gsz0 + gp, gs = 3 * nr * gsz;
@@ -1457,17 +1475,20 @@ constexpr int grp_pad(int map, int p, int nq, int g, int nr, int gsz0, int bufp,
       // accumulation reads
       if (map == 1 || map == 2) {
         // lane q g + row p + part reads row (column) `row` of its part's
-        // table: offset part gsz + (its exponent) xs
+        // table: offset part gsz + (its exponent) xs; per 16-lane group, the
+        // most distinct addresses on one bank pair
         const int xs = map == 1 ? NK : 1, nj = map == 1 ? NK : NI;
         for (int dim = 0; dim < 3; ++dim) {
-          for (int l = 0; l < 64; ++l) {
-            const int q = l / g, gl = l - q * g, row = gl / p, pp = gl - row * p;
-            int x = 0;
-            if (map == 1 && row < NE) x = dim == 0 ? re.lx[row] : (dim == 1 ? re.ly[row] : re.lz[row]);
-            if (map == 2 && row < NF) x = dim == 0 ? rf.lx[row] : (dim == 1 ? rf.ly[row] : rf.lz[row]);
-            addr[l] = q < nq ? q * slot + (dim * nr + pp) * gsz + x * xs : -1;
+          int cyc = 0;
+          for (int grp = 0; grp < 4; ++grp) {
+            int cnt[16] = {};
+            for (int j = 0; j < tn[dim][grp]; ++j)
+              ++cnt[(tq[dim][grp][j] * slot + (dim * nr + tp[dim][grp][j]) * gsz + tx[dim][grp][j] * xs) % 16];
+            int m = 0;
+            for (int k = 0; k < 16; ++k) m = cnt[k] > m ? cnt[k] : m;
+            cyc += m;
           }
-          cost += kt * nj * ((nr + p - 1) / p) * lds_cycles_b64(addr, 64);
+          cost += kt * nj * ((nr + p - 1) / p) * cyc;
         }
       } else {
         for (int k = 0; k * g < NE * NF; ++k)
