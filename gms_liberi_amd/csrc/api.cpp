@@ -95,6 +95,7 @@ struct liberi_ctx {
   double prim_cut = 1e-18;
   int row_tiles = 1;  // 0 never, 1 per class (autotuned or default), 2 wherever they fit
   bool autotune = true;
+  double max_device_bytes = 0.0;  // HBM budget of the handle (0: none)
   // basis
   bool initialized = false;
   int32_t nsh = 0, natoms = 0, num_bas = 0, mxgtot = 0, rank = 0, nprocs = 1;
@@ -158,6 +159,12 @@ static int dev_alloc(liberi_ctx *h, DevBuf &b, size_t bytes) {
   if (bytes == 0) {
     b.p = nullptr;
     return LIBERI_OK;
+  }
+  if (h->max_device_bytes > 0.0 && (double)h->stats.device_bytes + (double)bytes > h->max_device_bytes) {
+    b.bytes = 0;
+    return fail(LIBERI_ENOMEM, "device budget of the handle (LIBERI_OPT_MAX_DEVICE_BYTES = " +
+                                   std::to_string((int64_t)h->max_device_bytes) + ") exceeded by an allocation of " +
+                                   std::to_string(bytes) + " bytes");
   }
   hipError_t e = hipMalloc(&b.p, bytes);
   if (e != hipSuccess)
@@ -247,8 +254,9 @@ int liberi_c_create(liberi_ctx **ctx) {
 
 int liberi_c_cleanup(liberi_ctx *h) {
   if (!h) return fail(LIBERI_EARG, "liberi_c_cleanup: NULL handle");
-  if (!h->initialized) return LIBERI_OK;  // src/interface/liberi_interface.F90:235
-  (void)hipStreamSynchronize(h->stream);
+  // not set up: nothing to do (src/interface/liberi_interface.F90:235), but
+  // device data of a setup that failed part of the way is released as well
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
   release_device(h);
   return LIBERI_OK;
 }
@@ -299,6 +307,7 @@ int liberi_c_set_option(liberi_ctx *h, int32_t opt, double v) {
       h->row_tiles = (int)v;
       break;
     case LIBERI_OPT_GRAPH: h->use_graph = v != 0.0; break;
+    case LIBERI_OPT_MAX_DEVICE_BYTES: h->max_device_bytes = v; break;
     case LIBERI_OPT_AUTOTUNE:
       h->autotune = v != 0.0;
       // decide (and time) again at the next build, without earlier handles' choices
@@ -323,6 +332,7 @@ int liberi_c_get_option(liberi_ctx *h, int32_t opt, double *v) {
     case LIBERI_OPT_ROW_TILES: *v = h->row_tiles; break;
     case LIBERI_OPT_AUTOTUNE: *v = h->autotune; break;
     case LIBERI_OPT_GRAPH: *v = h->use_graph; break;
+    case LIBERI_OPT_MAX_DEVICE_BYTES: *v = h->max_device_bytes; break;
     default: return fail(LIBERI_EARG, "liberi_c_get_option: unknown option " + std::to_string(opt));
   }
   return LIBERI_OK;
@@ -437,22 +447,10 @@ static void append_pair(std::vector<PairRec> &pairs, std::vector<PrimRec> &prims
   pairs.push_back(r);
 }
 
-int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, int32_t mxgtot,
-                   const int32_t *ang_mom, const int32_t *contr_num, const int32_t *sh_loc, const int32_t *atom_num,
-                   const int32_t *atom_loc, const int32_t *start_bas, const int32_t *end_bas,
-                   const double *exponents, const double *cs, const double *cp, const double *cdc, const double *cf,
-                   const double *coords, const double *schwrz, int32_t my_rank, int32_t num_procs) {
-  (void)start_bas;  // stored but never read by the reference's hot path either
-  (void)end_bas;
-  if (!h) return fail(LIBERI_EARG, "liberi_c_setup: NULL handle");
-  if (!schwrz) return fail(LIBERI_EARG, "liberi_c_setup: NULL array");
-  if (num_procs < 1 || my_rank < 0 || my_rank >= num_procs)
-    return fail(LIBERI_EARG, "liberi_c_setup: need 0 <= my_rank < num_procs");
-  std::vector<HostShell> sh;
-  if (int rc0 = read_shells("liberi_c_setup", nsh, natoms, num_bas, mxgtot, ang_mom, contr_num, sh_loc, atom_num,
-                            atom_loc, exponents, cs, cp, cdc, cf, coords, sh))
-    return rc0;
-  if (h->initialized) liberi_c_cleanup(h);  // src/interface/liberi_interface.F90:133-136
+// Setup proper, on a handle whose device data is released; the caller
+// releases whatever it leaves behind when it fails part of the way through.
+static int setup_basis(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, int32_t mxgtot,
+                       const std::vector<HostShell> &sh, const double *schwrz, int32_t my_rank, int32_t num_procs) {
   h->nsh = nsh;
   h->natoms = natoms;
   h->num_bas = num_bas;
@@ -578,6 +576,8 @@ int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, 
   const int64_t ntri = (int64_t)num_bas * (num_bas + 1) / 2;
   if ((rc = dev_alloc(h, h->dens, ntri * sizeof(double)))) return rc;
   if ((rc = dev_alloc(h, h->aux, 2 * sizeof(double)))) return rc;
+  if (int e = launch_fill_zero((double *)h->aux.p, 2, h->stream))
+    return fail(LIBERI_EDEVICE, std::string("zero aux: ") + hipGetErrorString((hipError_t)e));
   if ((rc = dev_alloc(h, h->fock, ntri * sizeof(double)))) return rc;
 
   // ---- per class: pair pre-screen (host), quartet segments (device)
@@ -753,6 +753,34 @@ int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, 
   return LIBERI_OK;
 }
 
+int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, int32_t mxgtot,
+                   const int32_t *ang_mom, const int32_t *contr_num, const int32_t *sh_loc, const int32_t *atom_num,
+                   const int32_t *atom_loc, const int32_t *start_bas, const int32_t *end_bas,
+                   const double *exponents, const double *cs, const double *cp, const double *cdc, const double *cf,
+                   const double *coords, const double *schwrz, int32_t my_rank, int32_t num_procs) {
+  (void)start_bas;  // stored but never read by the reference's hot path either
+  (void)end_bas;
+  if (!h) return fail(LIBERI_EARG, "liberi_c_setup: NULL handle");
+  if (!schwrz) return fail(LIBERI_EARG, "liberi_c_setup: NULL array");
+  if (num_procs < 1 || my_rank < 0 || my_rank >= num_procs)
+    return fail(LIBERI_EARG, "liberi_c_setup: need 0 <= my_rank < num_procs");
+  std::vector<HostShell> sh;
+  if (int rc0 = read_shells("liberi_c_setup", nsh, natoms, num_bas, mxgtot, ang_mom, contr_num, sh_loc, atom_num,
+                            atom_loc, exponents, cs, cp, cdc, cf, coords, sh))
+    return rc0;
+  // a set-up handle is cleaned up first (src/interface/liberi_interface.F90:133-136);
+  // so is whatever an earlier setup that failed part of the way left behind
+  liberi_c_cleanup(h);
+  const int rc = setup_basis(h, nsh, natoms, num_bas, mxgtot, sh, schwrz, my_rank, num_procs);
+  if (rc != LIBERI_OK) {
+    const std::string msg = g_last_error;
+    (void)hipStreamSynchronize(h->stream);
+    release_device(h);
+    g_last_error = msg;
+  }
+  return rc;
+}
+
 // ------------------------------------------------------------------ build
 // Autotuner: for every class that has both a direct and a row-tiled kernel,
 // time each on this build's density (into a scratch Fock matrix, best of two
@@ -856,6 +884,16 @@ static int choose_variants(liberi_ctx *h, const BuildArgs &ba, hipStream_t st) {
   return LIBERI_OK;
 }
 
+// max |D| of the build into aux[0] and a zero FP32 quartet count into aux[1]
+// (the mixed form's switch reads them)
+static int enqueue_dmax(liberi_ctx *h, const double *dD, hipStream_t st) {
+  const int64_t ntri = (int64_t)h->num_bas * (h->num_bas + 1) / 2;
+  int e = launch_fill_zero((double *)h->aux.p, 2, st);
+  if (e == hipSuccess) e = launch_abs_max(dD, ntri, (double *)h->aux.p, st);
+  if (e != hipSuccess) return fail(LIBERI_EDEVICE, std::string("max |D|: ") + hipGetErrorString((hipError_t)e));
+  return LIBERI_OK;
+}
+
 // Zero F (and find max |D| for the mixed form), then every class's kernel.
 // With ns > 1 the classes are spread over the side streams, longest first
 // onto the least-loaded stream, forked from and joined back into st.
@@ -864,11 +902,8 @@ static int enqueue_build(liberi_ctx *h, const BuildArgs &ba, const double *dD, d
   // fock = 0 (src/liberi_driver.F90:64)
   if (int e = launch_fill_zero(dF, ntri, st))
     return fail(LIBERI_EDEVICE, std::string("zero F: ") + hipGetErrorString((hipError_t)e));
-  if (h->fp32_tau > 0.0) {
-    int e = launch_fill_zero((double *)h->aux.p, 2, st);
-    if (e == hipSuccess) e = launch_abs_max(dD, ntri, (double *)h->aux.p, st);
-    if (e != hipSuccess) return fail(LIBERI_EDEVICE, std::string("max |D|: ") + hipGetErrorString((hipError_t)e));
-  }
+  if (h->fp32_tau > 0.0)
+    if (int rc = enqueue_dmax(h, dD, st)) return rc;
   std::vector<int> order;
   for (int c = 0; c < kNumClasses; ++c)
     if (h->cls[c].active && (h->only_class < 0 || c == h->only_class)) order.push_back(c);
@@ -938,6 +973,9 @@ static int build_on_device(liberi_ctx *h, const double *dD, double *dF, hipStrea
     ba.col_bf[l] = (const int32_t *)h->col_bf[l].p;
     ba.ncol[l] = h->ncol[l];
   }
+  // the autotuner's timed launches see this build's max |D| (mixed form)
+  if (h->fp32_tau > 0.0)
+    if (int rc = enqueue_dmax(h, dD, st)) return rc;
   if (int rc = choose_variants(h, ba, st)) return rc;
   // the batched form: capture once (per D, F), replay afterwards.  Per-class
   // timing and single-class diagnostics run eagerly on st.

@@ -2082,11 +2082,11 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
   // lane searches the segments once and then steps forward, and they share
   // the bra pair (its records and density blocks stay in cache)
   // Workgroups resident at the same time take chunks spread over the whole
-  // quartet range (chunk = workgroup x 40009 mod grid, a permutation for any
-  // grid below that prime), so their atomics meet fewer common Fock rows.
+  // quartet range (chunk = workgroup x chunk_stride(grid) mod grid, a
+  // permutation), so their atomics meet fewer common Fock rows.
   const int64_t nq = cd.q_end - cd.q_begin;
   const int64_t per = ((nq + gridDim.x - 1) / gridDim.x + NQ - 1) / NQ * NQ;
-  const int64_t chunk = (int64_t)blockIdx.x * 40009 % gridDim.x;
+  const int64_t chunk = (int64_t)blockIdx.x * cd.chunk_stride % gridDim.x;
   const int64_t c0 = chunk * per, c1 = c0 + per < nq ? c0 + per : nq;
   int seg = -1;
   for (int64_t base = c0; base < c1; base += NQ) {
@@ -2390,7 +2390,9 @@ static int launch_one(const ClassDev &cd, const BuildArgs &ba, hipStream_t st, i
     constexpr int NQ = Grp<LA, LB, LC, LD>::NQ;
     const int64_t need = (n + NQ - 1) / NQ;
     const int64_t g = need < kGroupWorkgroups ? need : kGroupWorkgroups;
-    hipLaunchKernelGGL((eri_group_kernel<LA, LB, LC, LD>), dim3((unsigned)g), dim3(64), 0, st, cd, ba);
+    ClassDev cg = cd;
+    cg.chunk_stride = chunk_stride(g);
+    hipLaunchKernelGGL((eri_group_kernel<LA, LB, LC, LD>), dim3((unsigned)g), dim3(64), 0, st, cg, ba);
   } else {
     using S = Coop<LA, LB, LC, LD>;
     const int64_t g = n < nwg ? n : nwg;

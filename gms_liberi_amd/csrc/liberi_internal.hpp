@@ -91,6 +91,8 @@ struct ClassDev {
   // above); 1: the roles are swapped -- seg_bra holds ket pairs and ket_list
   // bra pairs -- so that a row tile fixes the ket pair (non-diagonal classes)
   int32_t transposed;
+  // group kernels: chunk_stride(grid) of the launch (set at launch)
+  int64_t chunk_stride;
 };
 
 struct BuildArgs {
@@ -137,6 +139,26 @@ constexpr int64_t kRowLdsBytes = 64 * 1024;
 constexpr int kCoopWorkgroups = 2048;
 // one-wave workgroups the group-per-quartet kernels use at most (persistent loop)
 constexpr int kGroupWorkgroups = 65536;
+
+// Order in which the workgroups of a persistent group-kernel launch take the
+// contiguous chunks of a class's quartet range: workgroup b takes chunk
+// b * s mod grid.  The stride s is coprime with the grid, so the map is a
+// permutation of 0 .. grid-1 for every grid size, and close to grid divided by
+// the golden ratio, so that workgroups resident at the same time (consecutive
+// b) work on distant parts of the range.
+constexpr int64_t chunk_stride(int64_t grid) {
+  if (grid <= 2) return 1;
+  int64_t s = (int64_t)((double)grid * 0.6180339887498949) | 1;
+  for (;; ++s) {  // ends at the latest at s = grid - 1, coprime with grid
+    int64_t a = grid, b = s;
+    while (b != 0) {
+      const int64_t t = a % b;
+      a = b;
+      b = t;
+    }
+    if (a == 1) return s;
+  }
+}
 
 // Schwarz values sqrt(max |(ab|ab)|) of the n shell pairs list[0..n) whose
 // shells have angular momenta la >= lb (A the la shell); out[i] for list[i].

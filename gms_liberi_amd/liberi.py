@@ -39,6 +39,7 @@ OPT_PRIM_CUTOFF = 7
 OPT_ROW_TILES = 8
 OPT_AUTOTUNE = 9
 OPT_GRAPH = 10
+OPT_MAX_DEVICE_BYTES = 11
 
 _STATUS = {1: "EARG", 2: "ESTATE", 3: "ESIZE", 4: "EDEVICE", 5: "ENOMEM"}
 
@@ -153,6 +154,8 @@ class Handle:
             a, p = _f64(getattr(b, f))
             keep.append(a)
             args.append(p)
+        # a failed setup leaves the handle released (not set up)
+        self.is_initialized = False
         _check(self._lib.liberi_c_setup(self._h, *args, my_rank, num_procs))
         self.num_bas = b.num_bas
         self.is_initialized = True

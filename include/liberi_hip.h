@@ -130,6 +130,11 @@ const char *liberi_c_last_error(void);
                                        the same D and F replay; 0: one stream, launched eagerly.
                                        Per-class timing and ONLY_CLASS builds run eagerly.  Read at
                                        each build; any option change drops the captured graph. */
+#define LIBERI_OPT_MAX_DEVICE_BYTES 11 /* HBM budget of the handle in bytes (0, the default: none).  An
+                                          allocation that would take the handle's device data past it
+                                          fails with LIBERI_ENOMEM; a setup that fails that way (or any
+                                          other way) leaves the handle released, ready for another
+                                          setup.  For callers that admit work by HBM footprint. */
 int liberi_c_set_option(liberi_ctx *ctx, int32_t option, double value);
 int liberi_c_get_option(liberi_ctx *ctx, int32_t option, double *value);
 

@@ -77,6 +77,41 @@ def test_primitive_pair_screen_is_below_summation_noise(name):
 
 
 @pytest.mark.gpu
+def test_setup_failing_part_way_leaves_a_reusable_handle(oracle):
+    """A setup that runs out of its device budget after some classes are set
+    up (LIBERI_OPT_MAX_DEVICE_BYTES) fails with ENOMEM and releases everything
+    it allocated; the same handle then sets up a smaller basis whose classes
+    are a subset of the failed one's, and builds it exactly as a fresh handle
+    and the oracle do."""
+    big = Case.load_npz(GOLD / "cfg_ethane.npz")
+    small = Case.load_npz(GOLD / "ref_h2o_ccpvdz.npz")
+    probe = liberi.Handle()
+    probe.setup(big.basis)
+    full = probe.stats()["device_bytes"]
+    probe.destroy()
+    h = liberi.Handle()
+    for frac in (0.5, 0.9):
+        h.set_option(liberi.OPT_MAX_DEVICE_BYTES, frac * full)
+        with pytest.raises(liberi.LiberiError) as e:
+            h.setup(big.basis)
+        assert e.value.code == 5 and "budget" in str(e.value)
+        assert h.stats()["device_bytes"] == 0
+        assert h.stats()["total_quartets"] == 0
+        with pytest.raises(liberi.LiberiError) as e:  # not set up
+            h.fock_build(big.density)
+        assert e.value.code == 2
+    h.set_option(liberi.OPT_MAX_DEVICE_BYTES, 0.0)
+    h.setup(small.basis)
+    f = h.fock_build(small.density)
+    st = h.stats()
+    h.destroy()
+    f_or, counts, _ = oracle.fock(small.basis, small.density)
+    assert np.max(np.abs(f - f_or)) <= 1e-10
+    assert np.max(np.abs(f - small.fock)) <= 1e-10
+    assert np.array_equal(st["quartets"], counts)
+
+
+@pytest.mark.gpu
 def test_basis_beyond_32_bit_packed_size_is_rejected():
     """num_bas(num_bas+1)/2 is a default integer in the reference's interface;
     the kernels index packed matrices in 32 bits, so setup refuses

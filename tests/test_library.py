@@ -117,3 +117,41 @@ def test_contraction_normalisation():
         p = e[:, None] + e[None]
         ov = (np.outer(w, w) * (np.pi / p) ** 1.5 * (0.5 / p) ** l).sum()
         assert abs(ov - 1.0) < 1e-12
+
+
+def test_group_chunk_order_is_a_permutation(tmp_path):
+    """Workgroup b of a group-kernel launch of `grid` workgroups takes chunk
+    b * chunk_stride(grid) mod grid: a bijection for every grid the launcher
+    can use (1 .. kGroupWorkgroups), including grids equal to or sharing
+    factors with any fixed multiplier."""
+    import shutil
+    import subprocess
+
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    src = tmp_path / "chunk.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include <vector>
+#include "liberi_internal.hpp"
+int main() {
+  std::vector<unsigned> seen(liberi::kGroupWorkgroups + 1, 0);
+  for (int64_t grid = 1; grid <= liberi::kGroupWorkgroups; ++grid) {
+    const int64_t s = liberi::chunk_stride(grid);
+    if (s < 1 || (grid > 1 && s >= grid)) { std::printf("bad stride %lld for %lld\n", (long long)s, (long long)grid); return 1; }
+    for (int64_t b = 0; b < grid; ++b) {
+      const int64_t c = b * s % grid;
+      if (seen[c] == (unsigned)grid) { std::printf("grid %lld: chunk %lld taken twice\n", (long long)grid, (long long)c); return 1; }
+      seen[c] = (unsigned)grid;
+    }
+  }
+  std::printf("ok\n");
+  return 0;
+}
+''')
+    exe = tmp_path / "chunk"
+    subprocess.run([gxx, "-O2", "-std=c++17", "-I", str(ROOT / "gms_liberi_amd" / "csrc"), str(src), "-o", str(exe)],
+                   check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout
