@@ -1131,39 +1131,11 @@ int liberi_c_schwarz(int32_t nsh, int32_t natoms, int32_t num_bas, int32_t mxgto
                      const int32_t *atom_loc, const double *exponents, const double *cs, const double *cp,
                      const double *cd, const double *cf, const double *coords, double *schwrz_int) {
   if (!schwrz_int) return fail(LIBERI_EARG, "liberi_c_schwarz: NULL output");
-  std::vector<HostShell> sh;
-  if (int rc = read_shells("liberi_c_schwarz", nsh, natoms, num_bas, mxgtot, ang_mom, contr_num, sh_loc, atom_num,
-                           atom_loc, exponents, cs, cp, cd, cf, coords, sh))
+  SchwarzStage s;
+  if (int rc = schwarz_stage("liberi_c_schwarz", nsh, natoms, num_bas, mxgtot, ang_mom, contr_num, sh_loc,
+                             atom_num, atom_loc, exponents, cs, cp, cd, cf, coords, s))
     return rc;
-  int dev = 0;
-  hipDeviceProp_t prop;
-  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
-    return fail(LIBERI_EDEVICE, "liberi_c_schwarz: no HIP device visible");
-  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
-    return fail(LIBERI_EDEVICE, std::string("liberi_c_schwarz: device is ") + prop.gcnArchName +
-                                    ", this library is built for gfx950 (MI355X)");
-  // every shell pair i >= j, A = the shell of higher angular momentum, grouped by (la, lb)
-  std::vector<PairRec> pairs;
-  std::vector<PrimRec> prims;
-  std::vector<int64_t> where;  // packed index of each pair, in pair order
-  std::vector<int32_t> by_l[16];
-  for (int i = 0; i < nsh; ++i)
-    for (int j = 0; j <= i; ++j) {
-      const int sA = sh[i].l >= sh[j].l ? i : j, sB = sA == i ? j : i;
-      by_l[sh[sA].l * 4 + sh[sB].l].push_back((int32_t)pairs.size());
-      where.push_back((int64_t)i * (i + 1) / 2 + j);
-      append_pair(pairs, prims, sh, nullptr, sA, sB);
-    }
-  if (prims.size() > (size_t)INT32_MAX || pairs.size() > (size_t)INT32_MAX)
-    return fail(LIBERI_EARG, "liberi_c_schwarz: basis too large for 32-bit pair indices");
-  std::vector<int32_t> list;
-  std::vector<size_t> off(17, 0);
-  for (int k = 0; k < 16; ++k) {
-    off[k] = list.size();
-    list.insert(list.end(), by_l[k].begin(), by_l[k].end());
-  }
-  off[16] = list.size();
-
+  if (int rc = schwarz_check_device("liberi_c_schwarz")) return rc;
   void *dp = nullptr, *dq = nullptr, *dl = nullptr, *dout = nullptr;
   hipStream_t st = nullptr;
   auto release = [&]() {
@@ -1171,29 +1143,90 @@ int liberi_c_schwarz(int32_t nsh, int32_t natoms, int32_t num_bas, int32_t mxgto
     for (void *p : {dp, dq, dl, dout})
       if (p) (void)hipFree(p);
   };
-  std::vector<double> out(list.size());
+  std::vector<double> out(s.list.size());
   hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipMalloc(&dp, pairs.size() * sizeof(PairRec));
-  if (e == hipSuccess) e = hipMalloc(&dq, prims.size() * sizeof(PrimRec));
-  if (e == hipSuccess) e = hipMalloc(&dl, list.size() * sizeof(int32_t));
-  if (e == hipSuccess) e = hipMalloc(&dout, list.size() * sizeof(double));
-  if (e == hipSuccess) e = hipMemcpyAsync(dp, pairs.data(), pairs.size() * sizeof(PairRec), hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(dq, prims.data(), prims.size() * sizeof(PrimRec), hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(dl, list.data(), list.size() * sizeof(int32_t), hipMemcpyHostToDevice, st);
-  for (int k = 0; k < 16 && e == hipSuccess; ++k) {
-    const int32_t n = (int32_t)(off[k + 1] - off[k]);
-    if (n > 0)
-      e = (hipError_t)launch_schwarz(k / 4, k % 4, (const PairRec *)dp, (const PrimRec *)dq,
-                                     (const int32_t *)dl + off[k], n, (double *)dout + off[k], st);
-  }
+  if (e == hipSuccess) e = hipMalloc(&dp, s.pairs.size() * sizeof(PairRec));
+  if (e == hipSuccess) e = hipMalloc(&dq, s.prims.size() * sizeof(PrimRec));
+  if (e == hipSuccess) e = hipMalloc(&dl, s.list.size() * sizeof(int32_t));
+  if (e == hipSuccess) e = hipMalloc(&dout, s.list.size() * sizeof(double));
+  if (e == hipSuccess)
+    e = (hipError_t)schwarz_enqueue(s, (PairRec *)dp, (PrimRec *)dq, (int32_t *)dl, (double *)dout, st);
   if (e == hipSuccess) e = hipMemcpyAsync(out.data(), dout, out.size() * sizeof(double), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   release();
   if (e != hipSuccess)
     return fail(e == hipErrorOutOfMemory ? LIBERI_ENOMEM : LIBERI_EDEVICE,
                 std::string("liberi_c_schwarz: ") + hipGetErrorString(e));
-  for (size_t k = 0; k < list.size(); ++k) schwrz_int[where[list[k]]] = out[k];
+  schwarz_scatter(s, out.data(), schwrz_int);
   return LIBERI_OK;
 }
 
 }  // extern "C"
+
+// Host staging shared by the two Schwarz entry points (liberi_c_schwarz here,
+// liberi_c_schwarz_noalloc in noalloc.hip).
+namespace liberi {
+
+int schwarz_stage(const char *who, int32_t nsh, int32_t natoms, int32_t num_bas, int32_t mxgtot,
+                  const int32_t *ang_mom, const int32_t *contr_num, const int32_t *sh_loc, const int32_t *atom_num,
+                  const int32_t *atom_loc, const double *exponents, const double *cs, const double *cp,
+                  const double *cd, const double *cf, const double *coords, SchwarzStage &s) {
+  std::vector<HostShell> sh;
+  if (int rc = read_shells(who, nsh, natoms, num_bas, mxgtot, ang_mom, contr_num, sh_loc, atom_num, atom_loc,
+                           exponents, cs, cp, cd, cf, coords, sh))
+    return rc;
+  // every shell pair i >= j, A = the shell of higher angular momentum, grouped by (la, lb)
+  std::vector<int32_t> by_l[16];
+  s.pairs.clear();
+  s.prims.clear();
+  s.where.clear();
+  s.list.clear();
+  for (int i = 0; i < nsh; ++i)
+    for (int j = 0; j <= i; ++j) {
+      const int sA = sh[i].l >= sh[j].l ? i : j, sB = sA == i ? j : i;
+      by_l[sh[sA].l * 4 + sh[sB].l].push_back((int32_t)s.pairs.size());
+      s.where.push_back((int64_t)i * (i + 1) / 2 + j);
+      append_pair(s.pairs, s.prims, sh, nullptr, sA, sB);
+    }
+  if (s.prims.size() > (size_t)INT32_MAX || s.pairs.size() > (size_t)INT32_MAX)
+    return fail(LIBERI_EARG, std::string(who) + ": basis too large for 32-bit pair indices");
+  for (int k = 0; k < 16; ++k) {
+    s.off[k] = s.list.size();
+    s.list.insert(s.list.end(), by_l[k].begin(), by_l[k].end());
+  }
+  s.off[16] = s.list.size();
+  return LIBERI_OK;
+}
+
+int schwarz_check_device(const char *who) {
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+    return fail(LIBERI_EDEVICE, std::string(who) + ": no HIP device visible");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(LIBERI_EDEVICE, std::string(who) + ": device is " + prop.gcnArchName +
+                                    ", this library is built for gfx950 (MI355X)");
+  return LIBERI_OK;
+}
+
+int schwarz_enqueue(const SchwarzStage &s, PairRec *dp, PrimRec *dq, int32_t *dl, double *dout, void *stream) {
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = hipMemcpyAsync(dp, s.pairs.data(), s.pairs.size() * sizeof(PairRec), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dq, s.prims.data(), s.prims.size() * sizeof(PrimRec), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dl, s.list.data(), s.list.size() * sizeof(int32_t), hipMemcpyHostToDevice, st);
+  for (int k = 0; k < 16 && e == hipSuccess; ++k) {
+    const int32_t n = (int32_t)(s.off[k + 1] - s.off[k]);
+    if (n > 0) e = (hipError_t)launch_schwarz(k / 4, k % 4, dp, dq, dl + s.off[k], n, dout + s.off[k], st);
+  }
+  return (int)e;
+}
+
+void schwarz_scatter(const SchwarzStage &s, const double *out, double *schwrz_int) {
+  for (size_t k = 0; k < s.list.size(); ++k) schwrz_int[s.where[s.list[k]]] = out[k];
+}
+
+int set_last_error(int code, const std::string &msg) { return fail(code, msg); }
+
+}  // namespace liberi

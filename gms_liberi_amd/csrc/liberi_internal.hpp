@@ -10,7 +10,10 @@
 // materialising them.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
+#include <string>
+#include <vector>
 
 namespace liberi {
 
@@ -164,6 +167,32 @@ constexpr int64_t chunk_stride(int64_t grid) {
 // shells have angular momenta la >= lb (A the la shell); out[i] for list[i].
 int launch_schwarz(int la, int lb, const PairRec *pairs, const PrimRec *prims, const int32_t *list, int32_t n,
                    double *out, void *stream);
+
+// Schwarz integrals of a whole basis, host side (api.cpp): every shell pair
+// i >= j with its primitive pairs (A = the shell of higher angular momentum),
+// the pairs grouped by (la, lb) in `list` (group la*4+lb is
+// list[off[k] .. off[k+1])), and per pair its packed index i(i+1)/2 + j.
+struct SchwarzStage {
+  std::vector<PairRec> pairs;
+  std::vector<PrimRec> prims;
+  std::vector<int64_t> where;
+  std::vector<int32_t> list;
+  size_t off[17];
+};
+// checked basis -> stage (LIBERI_OK or a status with the message set)
+int schwarz_stage(const char *who, int32_t nsh, int32_t natoms, int32_t num_bas, int32_t mxgtot,
+                  const int32_t *ang_mom, const int32_t *contr_num, const int32_t *sh_loc, const int32_t *atom_num,
+                  const int32_t *atom_loc, const double *exponents, const double *cs, const double *cp,
+                  const double *cd, const double *cf, const double *coords, SchwarzStage &s);
+// a gfx950 device is current (LIBERI_OK or LIBERI_EDEVICE with the message set)
+int schwarz_check_device(const char *who);
+// records and list uploaded into the given device buffers, one kernel per
+// (la, lb) group writing out[list position]; returns a hipError_t
+int schwarz_enqueue(const SchwarzStage &s, PairRec *dp, PrimRec *dq, int32_t *dl, double *dout, void *stream);
+// out (list order) -> schwrz_int (packed by shell pair)
+void schwarz_scatter(const SchwarzStage &s, const double *out, double *schwrz_int);
+// records the message of liberi_c_last_error and returns code
+int set_last_error(int code, const std::string &msg);
 
 // p[0..n) = 0 on the stream (a kernel, so it may sit in a captured graph)
 int launch_fill_zero(double *p, int64_t n, void *stream);

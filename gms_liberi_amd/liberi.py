@@ -86,15 +86,20 @@ def load_library(path: Path = LIB_PATH) -> C.CDLL:
     lib.liberi_c_fock.argtypes = [pdbl, pdbl, i64]
     lib.liberi_c_finalize.argtypes = []
     lib.liberi_c_schwarz.argtypes = [i32, i32, i32, i32] + [pi32] * 5 + [pdbl] * 7
+    lib.liberi_c_schwarz_noalloc.argtypes = [i32, i32, i32, i32] + [pi32] * 5 + [pdbl] * 7 + [vp]
     lib.liberi_c_fock_build_uhf.argtypes = [vp, pdbl, pdbl, pdbl, pdbl, i64]
     lib.liberi_c_fock_build_uhf_device.argtypes = [vp, vp, vp, vp, vp, i64, vp]
     _lib = lib
     return lib
 
 
-def schwarz(basis: Basis) -> np.ndarray:
+def schwarz(basis: Basis, noalloc: bool = False, stream: int | None = None) -> np.ndarray:
     """Schwarz integrals sqrt(max|(ab|ab)|) per shell pair, computed on the GPU,
-    in the layout of liberi_setup's schwrz_int (packed i(i-1)/2 + j, 1-based)."""
+    in the layout of liberi_setup's schwrz_int (packed i(i-1)/2 + j, 1-based).
+
+    noalloc=True calls liberi_c_schwarz_noalloc instead: no device allocation,
+    staging in the library's fixed device array, on `stream` (a hipStream_t
+    address; None = the default stream)."""
     lib = load_library()
     b = basis
     keep, args = [], [b.nsh, b.natoms, b.num_bas, b.mxgtot]
@@ -107,7 +112,10 @@ def schwarz(basis: Basis) -> np.ndarray:
         keep.append(a)
         args.append(p)
     out = np.zeros(b.nsh * (b.nsh + 1) // 2)
-    _check(lib.liberi_c_schwarz(*args, out.ctypes.data_as(C.POINTER(C.c_double))))
+    if noalloc:
+        _check(lib.liberi_c_schwarz_noalloc(*args, out.ctypes.data_as(C.POINTER(C.c_double)), stream))
+    else:
+        _check(lib.liberi_c_schwarz(*args, out.ctypes.data_as(C.POINTER(C.c_double))))
     return out
 
 

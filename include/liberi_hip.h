@@ -25,7 +25,7 @@ extern "C" {
 #define LIBERI_OK 0
 #define LIBERI_EARG 1    /* bad argument (NULL, negative size, inconsistent basis) */
 #define LIBERI_ESTATE 2  /* call out of order, e.g. fock build before setup */
-#define LIBERI_ESIZE 3   /* matrix size n != num_bas(num_bas+1)/2 */
+#define LIBERI_ESIZE 3   /* matrix size n != num_bas(num_bas+1)/2, or a basis above a fixed staging size */
 #define LIBERI_EDEVICE 4 /* HIP runtime failure or no usable gfx950 device */
 #define LIBERI_ENOMEM 5  /* device or host allocation failed */
 
@@ -65,6 +65,26 @@ int liberi_c_schwarz(int32_t nsh, int32_t natoms, int32_t num_bas, int32_t mxgto
                      const int32_t *atom_loc, const double *exponents, const double *contr_coef_s,
                      const double *contr_coef_p, const double *contr_coef_d, const double *contr_coef_f,
                      const double *coords, double *schwrz_int);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, serving loops that forbid hipMalloc
+ * and hipFree (hipFree synchronises the whole device, stalling the builds of
+ * other requests in flight): liberi_c_schwarz with the same arguments and
+ * results, which allocates no device memory and creates no stream.  Its
+ * records, pair list and results are staged in one library-owned device array
+ * of LIBERI_SCHWARZ_SCRATCH_BYTES per GPU, held until the process ends; a
+ * basis whose staging needs more (about 64 bytes per primitive pair plus 80
+ * per shell pair: some 900k primitive pairs) fails with LIBERI_ESIZE before
+ * any device work.  Runs on `stream` (a hipStream_t; NULL = the default
+ * stream) and returns when schwrz_int is written.  Every call on a GPU shares
+ * that GPU's array: calls that overlap on one GPU race, callers serialise
+ * them. */
+#define LIBERI_SCHWARZ_SCRATCH_BYTES (64LL << 20)
+int liberi_c_schwarz_noalloc(int32_t nsh, int32_t natoms, int32_t num_bas, int32_t mxgtot, const int32_t *ang_mom,
+                             const int32_t *contr_num, const int32_t *sh_loc, const int32_t *atom_num,
+                             const int32_t *atom_loc, const double *exponents, const double *contr_coef_s,
+                             const double *contr_coef_p, const double *contr_coef_d, const double *contr_coef_f,
+                             const double *coords, double *schwrz_int, void *stream);
 
 /* Replaces liberi_fock_build (src/interface/liberi_interface.F90:202-227) and
  * through it compute_integrals (src/liberi_driver.F90:52-167).  Host arrays of
