@@ -10,8 +10,12 @@ already in HBM (liberi_c_fock_build_device).
 Prints ONE JSON line: builds/s (value), contracted-quartet and
 primitive-quartet rates, the roofline of the dominant class kernel (FP64
 VALU), the accuracy of the build against the reference's own Fock matrix,
-and a CPU baseline: the reference CPU path (flang build, oracle/_ref) timed
-on this host over a bounded sample of the same workload.
+the host-array entry point's rate (D upload and F download included), the
+mixed-precision sweep over the FP32 switch tau, the time of each of 8 rank
+shares, the C-N chain (f classes) timed alone, a serving stream, and a CPU
+baseline: the reference CPU path (flang build, oracle/_ref) timed on this
+host on the whole workload (median of 3 builds), with its thread scaling on a
+12-atom piece of the cage.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -70,7 +74,7 @@ def timed_builds(h, D, F, n, stream, steps, hipmem):
     return start.elapsed_ms(end) / steps
 
 
-def gpu_leg(case, steps: int, warmup: int, tau: float, graph: bool = True):
+def gpu_leg(case, steps: int, warmup: int, taus=(), graph: bool = True, host_entry: bool = True):
     from gms_liberi_amd import census, hipmem, liberi
 
     h = liberi.Handle()
@@ -110,18 +114,33 @@ def gpu_leg(case, steps: int, warmup: int, tau: float, graph: bool = True):
     # plain FP64 without per-class events (the figure reported as value)
     ms_plain = timed_builds(h, D, F, n, stream, steps, hipmem)
     log(f"timed {steps} builds: {ms_plain:.3f} ms/build ({ms_per_step:.3f} with per-class events)")
-    mixed = None
-    if tau > 0:
+    # the host-array entry point (liberi_c_fock_build): D upload and F
+    # download over PCIe included, wall clock around each call
+    host = None
+    if host_entry:
+        f_host = np.empty(n)
+        h.fock_build(case.density, f_host)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            h.fock_build(case.density, f_host)
+        host_ms = (time.perf_counter() - t0) * 1e3 / steps
+        host = dict(ms=host_ms, max_abs_dF_vs_device=float(np.max(np.abs(f_host - fock))))
+        log(f"host-array entry point: {host_ms:.3f} ms/build (D upload and F download included)")
+    mixed = []
+    e64 = 0.5 * float(np.dot(case.density, fock))
+    for tau in taus:
         h.set_option(liberi.OPT_FP32_TAU, tau)
-        h.fock_build_device(D.address, F.address, n, stream.address)
+        h.fock_build_device(D.address, F.address, n, stream.address)  # (re)tunes the classes for this tau
         ms_mixed = timed_builds(h, D, F, n, stream, steps, hipmem)
         fm = F.to_host()
-        nf32 = h.stats()["fp32_quartets"]
-        h.set_option(liberi.OPT_FP32_TAU, 0.0)
-        e64 = 0.5 * float(np.dot(case.density, fock))
-        mixed = dict(tau=tau, ms=ms_mixed, fp32_quartets=int(nf32), max_abs_dF=float(np.max(np.abs(fm - fock))),
-                     dE=0.5 * float(np.dot(case.density, fm)) - e64)
-        log(f"mixed tau={tau:g}: {ms_mixed:.3f} ms/build, {nf32} FP32 quartets, dE {mixed['dE']:.2e}")
+        st_m = h.stats()
+        nf32 = st_m["fp32_quartets"]
+        mixed.append(dict(tau=tau, ms=ms_mixed, fp32_quartets=int(nf32),
+                          fp32_classes=[census.NAMES[c] for c in range(55) if st_m["class_fp32"][c]],
+                          max_abs_dF=float(np.max(np.abs(fm - fock))),
+                          dE=0.5 * float(np.dot(case.density, fm)) - e64))
+        log(f"mixed tau={tau:g}: {ms_mixed:.3f} ms/build, {nf32} FP32 quartets, dE {mixed[-1]['dE']:.2e}")
+    h.set_option(liberi.OPT_FP32_TAU, 0.0)
     h.destroy()
     stream.destroy()
     ms_per_step = ms_plain
@@ -138,58 +157,80 @@ def gpu_leg(case, steps: int, warmup: int, tau: float, graph: bool = True):
                  for c in np.argsort(-class_ms)[:8] if class_ms[c] > 0}
     return dict(ms_per_step=ms_per_step, fock=fock, stats=st, dom=dom, achieved=achieved,
                 dom_ms=float(class_ms[dom]), dom_flops=float(flops[dom]), total_flops=total_flops,
-                per_class=per_class, setup_s=setup_s, mixed=mixed)
+                per_class=per_class, setup_s=setup_s, mixed=mixed, host=host)
 
 
-def sampled_case(case, keep: float, seed: int = 7):
-    """Same workload with a seeded fraction of its shell pairs (each class's
-    largest pair always kept, so every pre-screen threshold is unchanged);
-    the other pairs get Schwarz value 0 and drop out in the pre-screen."""
-    import dataclasses
-
-    b = case.basis
-    rng = np.random.default_rng(seed)
-    sw = b.schwrz.copy()
-    mask = rng.random(sw.shape[0]) < keep
-    L = b.ang_mom - 1
-    i, j = np.tril_indices(b.nsh)
-    order = np.lexsort((j, i))
-    i, j = i[order], j[order]
-    cls = np.minimum(L[i], L[j]) * 4 + np.maximum(L[i], L[j])
-    for c in np.unique(cls):
-        idx = np.where(cls == c)[0]
-        mask[idx[np.argmax(sw[idx])]] = True
-    sw[~mask] = 0.0
-    return dataclasses.replace(case, basis=b.with_schwarz(sw)), float(mask.mean())
-
-
-def cpu_leg(case, keep: float):
-    """Reference CPU path (oracle/_ref, flang build) on a bounded sample."""
-    from gms_liberi_amd import liberi
-
+def ref_builds(case, threads: int, reps: int):
+    """Seconds of `reps` builds of the reference CPU path (oracle/_ref, flang
+    build of gms_libERI, OpenMP host fallback) on `threads` threads, after its
+    own setup, or None when it is not built."""
     exe = ROOT / "oracle" / "_ref" / "liberi_ref_run"
     if not exe.exists():
-        log("cpu baseline skipped: oracle/_ref not built")
         return None
-    samp, frac = sampled_case(case, keep)
-    h = liberi.Handle()  # the product counts the sample's surviving quartets (same rules, bit-exact)
-    h.setup(samp.basis)
-    q_sample = h.stats()["total_quartets"]
-    h.destroy()
-    threads = len(os.sched_getaffinity(0))
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", MALLOC_PERTURB_="255")
     with tempfile.TemporaryDirectory() as td:
-        cf, ff = Path(td) / "s.case", Path(td) / "s.fock"
-        samp.write_stream(cf)
-        log(f"cpu baseline: reference CPU path on {frac:.0%} of the shell pairs, {q_sample} quartets, "
-            f"{threads} threads")
-        r = subprocess.run([str(exe), str(cf), str(ff), "1"], env=env, capture_output=True, text=True, timeout=900)
+        cf, ff = Path(td) / "c.case", Path(td) / "c.fock"
+        case.write_stream(cf)
+        r = subprocess.run([str(exe), str(cf), str(ff), str(reps)], env=env, capture_output=True, text=True,
+                           timeout=1800)
         if r.returncode != 0:
-            log("cpu baseline failed: " + r.stderr[-500:])
+            log("reference CPU build failed: " + r.stderr[-500:])
             return None
         secs = [float(ln.split()[-1]) for ln in r.stdout.splitlines() if ln.startswith("build")]
-    t = min(secs)
-    return dict(seconds=t, quartets=int(q_sample), threads=threads, pair_fraction=frac)
+        fock = np.fromfile(ff, "<f8")
+    return secs, fock
+
+
+def cpu_leg(case, reps: int):
+    """The reference CPU path on the whole workload, all host threads, median
+    of `reps` builds; and its thread scaling on the 12-atom piece of the cage
+    (tests/golden/frag_c12.npz), one build on 1 thread and on all of them."""
+    from gms_liberi_amd.caseio import Case
+
+    threads = len(os.sched_getaffinity(0))
+    log(f"cpu baseline: reference CPU path, whole workload, {reps} builds on {threads} threads")
+    r = ref_builds(case, threads, reps)
+    if r is None:
+        log("cpu baseline skipped: oracle/_ref not built")
+        return None
+    secs, fock = r
+    out = dict(seconds=float(np.median(secs)), all_seconds=secs, threads=threads,
+               max_abs_dF_vs_fixture=float(np.max(np.abs(fock - case.fock))))
+    frag = Case.load_npz(ROOT / "tests" / "golden" / "frag_c12.npz")
+    scal = {}
+    for t in sorted({1, threads}):
+        rr = ref_builds(frag, t, 1)
+        if rr:
+            scal[str(t)] = rr[0][0]
+    out["scaling_frag_c12_s"] = scal
+    log(f"cpu baseline: {out['seconds']:.2f} s/build (median of {secs}); c12 piece by threads {scal}")
+    return out
+
+
+def rank_shares(case, nprocs: int, steps: int):
+    """Each of `nprocs` rank shares (my_rank/num_procs, the reference's split
+    of every class's quartet range) built alone on this GPU: the time of each
+    share, the imbalance max/mean, and the sum of the partial Fock matrices
+    against the one-rank build."""
+    from gms_liberi_amd import hipmem, liberi
+
+    n = case.basis.ntri
+    D = hipmem.DeviceArray.from_host(case.density)
+    F = hipmem.DeviceArray(n)
+    stream = hipmem.Stream()
+    ms, total = [], np.zeros(n)
+    for r in range(nprocs):
+        h = liberi.Handle()
+        h.setup(case.basis, r, nprocs)
+        h.fock_build_device(D.address, F.address, n, stream.address)  # autotuning
+        h.fock_build_device(D.address, F.address, n, stream.address)
+        ms.append(timed_builds(h, D, F, n, stream, steps, hipmem))
+        total += F.to_host()
+        h.destroy()
+    stream.destroy()
+    mean = float(np.mean(ms))
+    return dict(num_procs=nprocs, ms=[round(m, 3) for m in ms], max_over_mean=round(max(ms) / mean, 4),
+                sum_ms=round(float(np.sum(ms)), 3), max_abs_dF_sum_vs_reference=float(np.max(np.abs(total - case.fock))))
 
 
 def serving_leg(n: int, rate: float, seed: int, tau: float, slots: int):
@@ -243,15 +284,29 @@ def copy_calibration(mib: int) -> None:
     log(f"copied {mib} MiB device to device twice")
 
 
+def _mixed_summary(g, tau_main):
+    if not g["mixed"]:
+        return None
+    sweep = [{"tau": m["tau"], "value": round(1000.0 / m["ms"], 4), "speedup_vs_fp64": round(g["ms_per_step"] / m["ms"], 3),
+              "fp32_quartets": m["fp32_quartets"], "fp32_classes": m["fp32_classes"],
+              "max_abs_dF_vs_fp64": m["max_abs_dF"], "dE_vs_fp64": m["dE"]} for m in g["mixed"]]
+    main = next((m for m in sweep if m["tau"] == tau_main), sweep[0])
+    return {**main, "unit": "Fock builds/s", "sweep": sweep}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c60", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-pairs", type=float, default=0.3, help="shell-pair fraction of the CPU sample")
+    ap.add_argument("--cpu-reps", type=int, default=3, help="builds of the reference CPU path (median reported)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tau", type=float, default=1e-6, help="FP32 switch of the mixed-precision line (0: skip)")
+    ap.add_argument("--taus", default="1e-4,1e-5,1e-6,1e-7",
+                    help="FP32 switches of the mixed-precision sweep (comma separated; empty: skip)")
+    ap.add_argument("--tau", type=float, default=1e-6, help="FP32 switch of the mixed serving line")
+    ap.add_argument("--extra", default="cn_chain", help="further workloads timed alone (comma separated)")
+    ap.add_argument("--rank-shares", type=int, default=8, help="rank shares of the workload timed one by one (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=12, help="requests of the serving stream (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=1.0, help="arrivals per second of the serving stream")
     ap.add_argument("--serve-slots", type=int, default=4, help="requests on the GPU at once")
@@ -274,24 +329,41 @@ def main():
 
     case = Case.load_npz(ROOT / WORKLOADS[a.workload])
     log(f"workload {a.workload}: nbf={case.basis.num_bas} nsh={case.basis.nsh}")
-    g = gpu_leg(case, a.steps, a.warmup, a.tau, graph=not a.no_graph)
+    taus = [float(t) for t in a.taus.split(",") if t.strip()]
+    g = gpu_leg(case, a.steps, a.warmup, taus, graph=not a.no_graph)
     st = g["stats"]
     builds_per_s = 1000.0 / g["ms_per_step"]
     dF = float(np.max(np.abs(g["fock"] - case.fock)))
     dE = float(0.5 * np.dot(case.density, g["fock"] - case.fock))
 
+    extra = {}
+    for w in [x for x in a.extra.split(",") if x.strip() and x != a.workload]:
+        ce = Case.load_npz(ROOT / WORKLOADS[w])
+        log(f"extra workload {w}: nbf={ce.basis.num_bas} nsh={ce.basis.nsh}")
+        ge = gpu_leg(ce, a.steps, a.warmup, (), graph=not a.no_graph, host_entry=False)
+        extra[w] = {"workload": DESCR[w], "ms_per_build": round(ge["ms_per_step"], 4),
+                    "builds_per_s": round(1000.0 / ge["ms_per_step"], 4),
+                    "quartets_per_s": round(ge["stats"]["total_quartets"] * 1000.0 / ge["ms_per_step"], 1),
+                    "total_tflops": round(ge["total_flops"] / (ge["ms_per_step"] * 1e-3) * 1e-12, 3),
+                    "accuracy_vs_reference": {"max_abs_dF": float(np.max(np.abs(ge["fock"] - ce.fock))),
+                                              "dE_2e": float(0.5 * np.dot(ce.density, ge["fock"] - ce.fock))},
+                    "top_kernels": ge["per_class"]}
+    shares = rank_shares(case, a.rank_shares, a.steps) if a.rank_shares > 1 else None
+    if shares:
+        log(f"rank shares of {a.rank_shares}: {shares['ms']} ms, max/mean {shares['max_over_mean']}")
     serve = serving_leg(a.serve_requests, a.serve_rate, 5, a.tau, a.serve_slots) if a.serve_requests > 0 else None
-    cpu = None if a.no_cpu else cpu_leg(case, a.cpu_pairs)
+    cpu = None if a.no_cpu else cpu_leg(case, a.cpu_reps)
     cpu_base = None
     if cpu:
         q_total = st["total_quartets"]
-        cpu_base = {"value": round(cpu["quartets"] / cpu["seconds"] / q_total, 6), "unit": "Fock builds/s",
+        cpu_base = {"value": round(1.0 / cpu["seconds"], 6), "unit": "Fock builds/s",
                     "cores": cpu["threads"], "kind": "reference",
-                    "sample": (f"reference CPU path (flang build of gms_libERI, OpenMP host fallback) on the same "
-                               f"case restricted to a seeded {cpu['pair_fraction']:.0%} of its shell pairs: "
-                               f"{cpu['quartets']} of {q_total} surviving quartets in {cpu['seconds']:.2f} s; "
-                               f"builds/s = sample quartets/s / full quartet count"),
-                    "quartets_per_s": round(cpu["quartets"] / cpu["seconds"], 1)}
+                    "sample": (f"the whole workload: reference CPU path (flang build of gms_libERI, OpenMP host "
+                               f"fallback) on {cpu['threads']} threads, median of {len(cpu['all_seconds'])} builds "
+                               f"{[round(x, 2) for x in cpu['all_seconds']]} s; max|F - fixture| "
+                               f"{cpu['max_abs_dF_vs_fixture']:.1e}"),
+                    "quartets_per_s": round(q_total / cpu["seconds"], 1),
+                    "thread_scaling_frag_c12_s": cpu["scaling_frag_c12_s"]}
 
     from gms_liberi_amd import census
 
@@ -324,11 +396,12 @@ def main():
                      "traffic_detail": traffic,
                      "kernel_ms": round(g["dom_ms"], 4), "kernel_flops": g["dom_flops"]},
         "top_kernels": g["per_class"],
-        "mixed_precision": None if not g["mixed"] else {
-            "tau": g["mixed"]["tau"], "value": round(1000.0 / g["mixed"]["ms"], 4), "unit": "Fock builds/s",
-            "speedup_vs_fp64": round(g["ms_per_step"] / g["mixed"]["ms"], 3),
-            "fp32_quartets": g["mixed"]["fp32_quartets"], "max_abs_dF_vs_fp64": g["mixed"]["max_abs_dF"],
-            "dE_vs_fp64": g["mixed"]["dE"]},
+        "host_entry": None if not g["host"] else {
+            "value": round(1000.0 / g["host"]["ms"], 4), "unit": "Fock builds/s", "ms_per_build": round(g["host"]["ms"], 4),
+            "note": "liberi_c_fock_build on host arrays: D upload and F download included"},
+        "mixed_precision": _mixed_summary(g, a.tau),
+        "rank_shares": shares,
+        **extra,
         "serving": serve,
         "cpu_baseline": cpu_base,
     }

@@ -192,7 +192,15 @@ def cluster(natoms: int, seed: int) -> Case:
     return build_case(f"cluster{natoms}_{seed}", syms, pts, _heavy, max(1, nelec // 2), seed + 1000)
 
 
-GENERATORS = {"water": water, "ethane": ethane, "c60": c60, "cn_chain": cn_chain, "c12": c60_fragment}
+def cluster12(seed: int = 3) -> Case:
+    """A 12-atom request of the serving stream's recipe (fixture size)."""
+    case = cluster(12, seed)
+    case.name = "cluster12"
+    return case
+
+
+GENERATORS = {"water": water, "ethane": ethane, "c60": c60, "cn_chain": cn_chain, "c12": c60_fragment,
+              "cluster12": cluster12}
 
 
 def make_case(name: str, schwarz=None) -> Case:

@@ -9,7 +9,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -70,7 +72,7 @@ static int fail(int code, const std::string &msg) {
                   std::string(#expr) + ": " + hipGetErrorString(e_));                                   \
   } while (0)
 
-static void set_tuned_choice(int c, int v);
+static void forget_tuned_choices();
 
 // --------------------------------------------------------------- handle
 struct DevBuf {
@@ -82,6 +84,7 @@ struct ClassState {
   bool active = false;
   bool row_ok = false;  // a row-tiled kernel exists and fits in LDS
   int variant = -1;     // digestion chosen for the class: 0 direct, 1 row-tiled (-1: not yet)
+  int fp32 = -1;        // mixed form: 1 FP32 allowed, 0 not (-1: not yet chosen for the current tau)
   float ms = -1.f;      // device time of the class's launch, measured at the first build
   int64_t nquart = 0, nprimq = 0;
   int32_t nseg = 0;
@@ -299,7 +302,11 @@ int liberi_c_set_option(liberi_ctx *h, int32_t opt, double v) {
     case LIBERI_OPT_CLASS_TIMING: h->class_timing = v != 0.0; break;
     case LIBERI_OPT_SCHWARZ_CUTOFF: h->schwarz_cut = v; break;
     case LIBERI_OPT_VALUE_CUTOFF: h->value_cut = v; break;
-    case LIBERI_OPT_FP32_TAU: h->fp32_tau = v; break;
+    case LIBERI_OPT_FP32_TAU:
+      if (v != h->fp32_tau)
+        for (auto &c : h->cls) c.fp32 = -1;  // chosen again at the next build
+      h->fp32_tau = v;
+      break;
     case LIBERI_OPT_EXACT_LARGE_T: h->exact_large_t = v != 0.0; break;
     case LIBERI_OPT_PRIM_CUTOFF: h->prim_cut = v; break;
     case LIBERI_OPT_ROW_TILES:
@@ -311,8 +318,8 @@ int liberi_c_set_option(liberi_ctx *h, int32_t opt, double v) {
     case LIBERI_OPT_AUTOTUNE:
       h->autotune = v != 0.0;
       // decide (and time) again at the next build, without earlier handles' choices
-      for (auto &c : h->cls) c.variant = -1, c.ms = -1.f;
-      set_tuned_choice(-1, -1);
+      for (auto &c : h->cls) c.variant = -1, c.fp32 = -1, c.ms = -1.f;
+      forget_tuned_choices();
       break;
     default: return fail(LIBERI_EARG, "liberi_c_set_option: unknown option " + std::to_string(opt));
   }
@@ -346,6 +353,7 @@ int liberi_c_get_stats(liberi_ctx *h, liberi_stats *out) {
     const ClassState &cs_ = h->cls[c];
     const bool row = cs_.row_ok && (h->row_tiles == 2 || (h->row_tiles == 1 && cs_.variant == 1));
     out->class_variant[c] = !cs_.active || cs_.variant < 0 ? -1 : row ? 1 + cs_.dev.transposed : 0;
+    out->class_fp32[c] = cs_.active && h->fp32_tau > 0.0 && cs_.fp32 == 1;
   }
   return LIBERI_OK;
 }
@@ -792,32 +800,33 @@ int liberi_c_setup(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_bas, 
 constexpr int64_t kTuneMinQuartets = 20000;
 constexpr int kSideStreams = 4;
 
-// Choices made so far in this process, per class: later handles (a request
-// loop sets up one per molecule) reuse them instead of timing again.
+// Choices made so far in this process: later handles (a request loop sets up
+// one per molecule) reuse them instead of timing again, but only for a class
+// of about the same size -- the key is the class and the power of two of its
+// quartet count -- and, for the FP32 choice, the same switch tau.
 static std::mutex g_tune_mu;
-static int g_tuned[kNumClasses];
-static bool g_tuned_init = false;
+static std::map<std::tuple<int, int, double>, int> g_tuned;  // (class, size, tau or -1) -> choice
 
-static int tuned_choice(int c) {
-  std::lock_guard<std::mutex> lk(g_tune_mu);
-  if (!g_tuned_init) {
-    for (int &v : g_tuned) v = -1;
-    g_tuned_init = true;
-  }
-  return g_tuned[c];
+static int size_bucket(int64_t nq) {
+  int b = 0;
+  while (nq > 1) nq >>= 1, ++b;
+  return b;
 }
 
-static void set_tuned_choice(int c, int v) {
+static int tuned_choice(int c, int64_t nq, double tau) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  if (!g_tuned_init) {
-    for (int &x : g_tuned) x = -1;
-    g_tuned_init = true;
-  }
-  if (c < 0) {
-    for (int &x : g_tuned) x = -1;
-  } else {
-    g_tuned[c] = v;
-  }
+  auto it = g_tuned.find({c, size_bucket(nq), tau});
+  return it == g_tuned.end() ? -1 : it->second;
+}
+
+static void set_tuned_choice(int c, int64_t nq, double tau, int v) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned[{c, size_bucket(nq), tau}] = v;
+}
+
+static void forget_tuned_choices() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned.clear();
 }
 
 // Whether class c runs its row-tiled kernel in this build.
@@ -829,7 +838,8 @@ static bool class_row_mode(const liberi_ctx *h, int c) {
 
 // One launch of class c (row-tiled or direct) into the scratch Fock matrix,
 // timed with events on st.
-static int time_class(liberi_ctx *h, const BuildArgs &ba, hipStream_t st, int c, bool row_mode, float &ms) {
+static int time_class(liberi_ctx *h, const BuildArgs &ba, hipStream_t st, int c, bool row_mode, bool fp32,
+                      float &ms) {
   const int64_t ntri = (int64_t)h->num_bas * (h->num_bas + 1) / 2;
   if (!h->tune_F.p)
     if (int rc = dev_alloc(h, h->tune_F, ntri * sizeof(double))) return rc;
@@ -839,6 +849,7 @@ static int time_class(liberi_ctx *h, const BuildArgs &ba, hipStream_t st, int c,
   bt.F = (double *)h->tune_F.p;
   ClassDev cd = h->cls[c].dev;
   cd.row_mode = row_mode;
+  cd.fp32 = fp32;
   HIP_TRY(hipEventRecord(h->tev[0], st));
   if (int e = launch_class(c, cd, bt, st, kCoopWorkgroups))
     return fail(LIBERI_EDEVICE, std::string("timing ") + kClasses[c].name + ": " + hipGetErrorString((hipError_t)e));
@@ -859,7 +870,7 @@ static int choose_variants(liberi_ctx *h, const BuildArgs &ba, hipStream_t st) {
       cs_.variant = dflt;
       continue;
     }
-    if (const int prev = tuned_choice(c); prev >= 0) {
+    if (const int prev = tuned_choice(c, nq, -1.0); prev >= 0) {
       cs_.variant = prev;
       continue;
     }
@@ -867,19 +878,50 @@ static int choose_variants(liberi_ctx *h, const BuildArgs &ba, hipStream_t st) {
     for (int rep = 0; rep < 4; ++rep) {
       const int v = rep & 1;
       float ms = 0.f;
-      if (int rc = time_class(h, ba, st, c, v == 1, ms)) return rc;
+      if (int rc = time_class(h, ba, st, c, v == 1, false, ms)) return rc;
       best[v] = std::min(best[v], ms);
     }
     cs_.variant = best[1] < best[0] ? 1 : 0;
     cs_.ms = std::min(best[0], best[1]);
-    set_tuned_choice(c, cs_.variant);
+    set_tuned_choice(c, nq, -1.0, cs_.variant);
+  }
+  // mixed form: FP32 stays on only in the classes where it is measured
+  // faster than FP64 by at least kFp32Gain (without packed FP32 an FP32
+  // instruction costs what an FP64 one does, so most classes lose by the
+  // conversions); the lane-per-quartet classes have an FP32 path
+  constexpr float kFp32Gain = 0.02f;
+  for (int c = 0; c < kNumClasses; ++c) {
+    ClassState &cs_ = h->cls[c];
+    if (!cs_.active || cs_.fp32 >= 0) continue;
+    const int64_t nq = cs_.dev.q_end - cs_.dev.q_begin;
+    if (!(h->fp32_tau > 0.0) || class_kernel_kind(c) != 0) {
+      cs_.fp32 = h->fp32_tau > 0.0 ? 0 : -1;
+      continue;
+    }
+    if (!h->autotune || nq < kTuneMinQuartets || (h->only_class >= 0 && c != h->only_class)) {
+      cs_.fp32 = 1;  // untimed: the form as asked for
+      continue;
+    }
+    if (const int prev = tuned_choice(c, nq, h->fp32_tau); prev >= 0) {
+      cs_.fp32 = prev;
+      continue;
+    }
+    float best[2] = {1e30f, 1e30f};
+    for (int rep = 0; rep < 4; ++rep) {
+      const int v = rep & 1;
+      float ms = 0.f;
+      if (int rc = time_class(h, ba, st, c, class_row_mode(h, c), v == 1, ms)) return rc;
+      best[v] = std::min(best[v], ms);
+    }
+    cs_.fp32 = best[1] < (1.f - kFp32Gain) * best[0] ? 1 : 0;
+    set_tuned_choice(c, nq, h->fp32_tau, cs_.fp32);
   }
   // one timed launch of every class not timed above: the measured times
   // order the classes over the side streams
   for (int c = 0; c < kNumClasses; ++c) {
     ClassState &cs_ = h->cls[c];
     if (!cs_.active || cs_.ms >= 0.f || (h->only_class >= 0 && c != h->only_class)) continue;
-    if (int rc = time_class(h, ba, st, c, class_row_mode(h, c), cs_.ms)) return rc;
+    if (int rc = time_class(h, ba, st, c, class_row_mode(h, c), cs_.fp32 == 1, cs_.ms)) return rc;
   }
   return LIBERI_OK;
 }
@@ -938,6 +980,7 @@ static int enqueue_build(liberi_ctx *h, const BuildArgs &ba, const double *dD, d
     }
     ClassDev cd = h->cls[c].dev;
     cd.row_mode = class_row_mode(h, c);
+    cd.fp32 = h->cls[c].fp32 == 1;
     int e = launch_class(c, cd, ba, cs, kCoopWorkgroups);
     if (e != hipSuccess)
       return fail(LIBERI_EDEVICE, std::string("launch ") + kClasses[c].name + ": " + hipGetErrorString((hipError_t)e));

@@ -331,11 +331,13 @@ __device__ __forceinline__ void hrr_ket(T *buf0, T *buf1, T cdx, T cdy, T cdz) {
 }
 
 // Mixed precision: quartets whose Schwarz bound times max|D| is below tau
-// may be evaluated in FP32 (tau = 0: never).  The form is chosen per wave --
-// FP32 only when the quartet of every active lane qualifies -- so that a wave
-// never runs both forms of the primitive loop.
-__device__ __forceinline__ bool mixed_wave(const BuildArgs &ba, const PairRec &bra, const PairRec &ket) {
-  if (!(ba.fp32_tau > 0.0)) return false;
+// may be evaluated in FP32 (tau = 0: never), in the classes the build enables
+// it for.  The form is chosen per wave -- FP32 only when the quartet of every
+// active lane qualifies -- so that a wave never runs both forms of the
+// primitive loop.
+__device__ __forceinline__ bool mixed_wave(const ClassDev &cd, const BuildArgs &ba, const PairRec &bra,
+                                           const PairRec &ket) {
+  if (!cd.fp32 || !(ba.fp32_tau > 0.0)) return false;
   const bool lo = bra.x * ket.x * ba.dmax[0] < ba.fp32_tau;
   return __ballot(!lo) == 0;
 }
@@ -484,7 +486,7 @@ __global__ __launch_bounds__(256) void eri_small_kernel(ClassDev cd, BuildArgs b
   const PairRec bra = ba.pairs[ib];
   const PairRec ket = ba.pairs[ik];
   double res[S::NABCD];
-  if (mixed_wave(ba, bra, ket)) {
+  if (mixed_wave(cd, ba, bra, ket)) {
     const uint64_t m = __ballot(true);
     if (__lane_id() == __ffsll((unsigned long long)m) - 1)
       atomicAdd((unsigned long long *)ba.fp32_count, (unsigned long long)__popcll(m));
@@ -726,7 +728,7 @@ __global__ __launch_bounds__(256) void eri_row_kernel(ClassDev cd, BuildArgs ba)
     const PairRec &bra = TR ? mov : fix;
     const PairRec &ket = TR ? fix : mov;
     double v[S::NABCD];
-    if (mixed_wave(ba, bra, ket)) {
+    if (mixed_wave(cd, ba, bra, ket)) {
       ++nlow;
       small_integrals<float, LA, LB, LC, LD, !TR>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, v, tab,
                                              rys_lds_stride(S::NR));

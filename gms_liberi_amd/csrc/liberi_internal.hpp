@@ -96,6 +96,9 @@ struct ClassDev {
   int32_t transposed;
   // group kernels: chunk_stride(grid) of the launch (set at launch)
   int64_t chunk_stride;
+  // 1: the mixed form may evaluate this class's quartets in FP32 (chosen per
+  // class at the first build under a given switch tau)
+  int32_t fp32;
 };
 
 struct BuildArgs {

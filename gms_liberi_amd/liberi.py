@@ -54,7 +54,8 @@ class Stats(C.Structure):
     _fields_ = [("quartets", C.c_int64 * NCLASS), ("prim_quartets", C.c_int64 * NCLASS),
                 ("total_quartets", C.c_int64), ("fp32_quartets", C.c_int64), ("last_build_ms", C.c_double),
                 ("device_bytes", C.c_int64), ("class_ms", C.c_double * NCLASS), ("prim_pairs", C.c_int64),
-                ("prim_pairs_dropped", C.c_int64), ("class_variant", C.c_int32 * NCLASS)]
+                ("prim_pairs_dropped", C.c_int64), ("class_variant", C.c_int32 * NCLASS),
+                ("class_fp32", C.c_int32 * NCLASS)]
 
 
 _lib = None
@@ -227,7 +228,8 @@ class Handle:
                 "last_build_ms": s.last_build_ms, "device_bytes": s.device_bytes,
                 "class_ms": np.array(s.class_ms[:], np.float64), "prim_pairs": s.prim_pairs,
                 "prim_pairs_dropped": s.prim_pairs_dropped,
-                "class_variant": np.array(s.class_variant[:], dtype=np.int32)}
+                "class_variant": np.array(s.class_variant[:], dtype=np.int32),
+                "class_fp32": np.array(s.class_fp32[:], dtype=np.int32)}
 
 
 # ---------------------------------------------- reference-named functions

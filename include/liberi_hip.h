@@ -123,11 +123,14 @@ const char *liberi_c_last_error(void);
 #define LIBERI_OPT_VALUE_CUTOFF 2   /* |integral| skip threshold, default 5.00000000000000018e-11 */
 #define LIBERI_OPT_FP32_TAU 3       /* mixed precision: quartets of the lane-per-quartet classes whose
                                        Schwarz bound x_ab x_cd times max|D| is below tau are evaluated
-                                       in FP32 (Rys roots/weights and all Fock accumulation stay FP64);
-                                       0 = plain FP64 (default).  Read at each build. */
+                                       in FP32 (Rys roots/weights and all Fock accumulation stay FP64),
+                                       in those classes where the first build under this tau measures
+                                       FP32 faster (liberi_stats.class_fp32); 0 = plain FP64 (default).
+                                       Read at each build. */
 #define LIBERI_OPT_EXACT_LARGE_T 4  /* 0 (default): large-argument forms from the reference's switch
                                        points (T > 25 for its Boys-based classes, X >= 37..65 by root
-                                       count for its Rys classes); 1: only where exact to double */
+                                       count for its Rys classes); 1: only where exact to double.
+                                       Read at setup. */
 #define LIBERI_OPT_CLASS_TIMING 5   /* 1: time every class launch with HIP events (stats.class_ms) */
 #define LIBERI_OPT_ONLY_CLASS 6     /* >= 0: build only this class (diagnostics); -1 (default): all */
 #define LIBERI_OPT_PRIM_CUTOFF 7    /* drop primitive pairs whose share of any integral they can
@@ -174,6 +177,9 @@ typedef struct {
   int32_t class_variant[LIBERI_NCLASS]; /* digestion used per class: 0 direct atomics, 1 row tiles
                                            over bra pairs, 2 row tiles over ket pairs; -1 class
                                            inactive or not yet built */
+  int32_t class_fp32[LIBERI_NCLASS];    /* 1: the mixed form (LIBERI_OPT_FP32_TAU > 0) evaluates this
+                                           class's qualifying quartets in FP32, chosen per class from
+                                           measured time at the first build under that tau; 0: FP64 */
 } liberi_stats;
 int liberi_c_get_stats(liberi_ctx *ctx, liberi_stats *out);
 

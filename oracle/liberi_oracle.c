@@ -83,6 +83,14 @@ int oracle_nclass(void) { return ORACLE_NCLASS; }
 const char *oracle_class_name(int c) {
   return (c >= 0 && c < ORACLE_NCLASS) ? CLASSES[c].name : "";
 }
+int oracle_class_def(int c, int def[4]) {
+  if (c < 0 || c >= ORACLE_NCLASS) return -1;
+  def[0] = CLASSES[c].bra;
+  def[1] = CLASSES[c].ket;
+  def[2] = CLASSES[c].cmp;
+  def[3] = CLASSES[c].ps;
+  return 0;
+}
 
 /* ------------------------------------------------------- cartesian tables */
 /* GAMESS component order: p x,y,z; d xx,yy,zz,xy,xz,yz; f xxx,yyy,zzz,xxy,
@@ -120,6 +128,30 @@ double oracle_boys(int m, double T) {
     if (term < sum * 1e-21L) break;
   }
   return (double)(sum * expl(-(long double)T));
+}
+
+/* The reference's large-argument form: past its switch point each kernel
+ * takes F_m = (2m-1)!!/(2T)^m sqrt(pi/4T) instead of its tabulated Boys
+ * values or Rys roots (src/rhf/rot_axis/int0122.F90:493-501 and the same
+ * branch in every rotated-axis kernel, src/rhf/eric/int0030_ericgen.F90:300-311;
+ * its Rys kernels switch to asymptotic roots, whose moments are these values,
+ * e.g. src/rhf/rot_axis/int2222_rysgen.F90:642).  Switch points: T > 25
+ * (t_max, src/liberi_boys.F90:22) for the rotated-axis and ERIC classes,
+ * X >= 37, 43, 49, 55, 60, 65 for the Rys classes with 2..7 roots. */
+typedef struct {
+  double t;   /* switch point (INFINITY: exact everywhere) */
+  int ge;     /* 1: asymptotic when T >= t; 0: when T > t */
+} boys_switch;
+
+static double boys_asymptotic(int m, double T) {
+  long double f = sqrtl(3.14159265358979323846264338327950288L / (4.0L * T));
+  for (int k = 0; k < m; ++k) f *= (2 * k + 1) / (2.0L * T);
+  return (double)f;
+}
+
+static double boys_sw(int m, double T, const boys_switch *sw) {
+  if (sw && (sw->ge ? T >= sw->t : T > sw->t)) return boys_asymptotic(m, T);
+  return oracle_boys(m, T);
 }
 
 /* --------------------------------------------------------------- shells */
@@ -171,14 +203,14 @@ static void hermite_e(int la, int lb, double p, double xpa, double xpb, double E
 }
 
 /* R^0_{tuv}, t+u+v <= L, stored R[(t*(L+1)+u)*(L+1)+v] */
-static void hermite_r(int L, double alpha, const double pq[3], double *R, double *work) {
+static void hermite_r(int L, double alpha, const double pq[3], double *R, double *work, const boys_switch *sw) {
   const int n1 = L + 1;
   double T = alpha * (pq[0] * pq[0] + pq[1] * pq[1] + pq[2] * pq[2]);
   /* work[n][t][u][v] */
 #define W(n, t, u, v) work[(((n) * n1 + (t)) * n1 + (u)) * n1 + (v)]
   double m2a = -2.0 * alpha, fac = 1.0;
   for (int n = 0; n <= L; ++n) {
-    W(n, 0, 0, 0) = fac * oracle_boys(n, T);
+    W(n, 0, 0, 0) = fac * boys_sw(n, T, sw);
     fac *= m2a;
   }
   for (int n = L - 1; n >= 0; --n) {
@@ -208,7 +240,8 @@ static void hermite_r(int L, double alpha, const double pq[3], double *R, double
 }
 
 /* normalized cartesian block of shells A,B,C,D (GAMESS order) */
-static void md_quartet(const shell_t *A, const shell_t *B, const shell_t *C, const shell_t *D, double *out) {
+static void md_quartet(const shell_t *A, const shell_t *B, const shell_t *C, const shell_t *D, double *out,
+                       const boys_switch *sw) {
   const int na = NCART[A->l], nb = NCART[B->l], nc = NCART[C->l], nd = NCART[D->l];
   const int nab = na * nb, ncd = nc * nd, n = nab * ncd;
   const int L = A->l + B->l + C->l + D->l, n1 = L + 1;
@@ -258,7 +291,7 @@ static void md_quartet(const shell_t *A, const shell_t *B, const shell_t *C, con
             pq[x] = P[x] - Q[x];
           }
           const double alpha = p * q / (p + q);
-          hermite_r(L, alpha, pq, R, work);
+          hermite_r(L, alpha, pq, R, work, sw);
           hermite_e(C->l, D->l, q, Q[0] - C->ctr[0], Q[0] - D->ctr[0], Fx);
           hermite_e(C->l, D->l, q, Q[1] - C->ctr[1], Q[1] - D->ctr[1], Fy);
           hermite_e(C->l, D->l, q, Q[2] - C->ctr[2], Q[2] - D->ctr[2], Fz);
@@ -323,7 +356,7 @@ int oracle_shell_quartet(const oracle_basis *b, int i, int j, int k, int l, doub
   get_shell(b, k, &C);
   get_shell(b, l, &D);
   if (A.l > LMAX || B.l > LMAX || C.l > LMAX || D.l > LMAX) return -2;
-  md_quartet(&A, &B, &C, &D, out);
+  md_quartet(&A, &B, &C, &D, out, NULL);
   return 0;
 }
 
@@ -341,7 +374,7 @@ int oracle_schwarz(const oracle_basis *b, double *sw) {
         bad = 1;
         continue;
       }
-      md_quartet(&A, &B, &A, &B, blk);
+      md_quartet(&A, &B, &A, &B, blk, NULL);
       const int na = NCART[A.l], nb = NCART[B.l];
       double mx = 0.0;
       for (int a = 0; a < na; ++a)
@@ -552,8 +585,30 @@ int64_t oracle_fock(const oracle_basis *b, const double *Dm, double *F, double c
   return oracle_fock_classes(b, Dm, F, cut, counts, stride, nthreads, NULL);
 }
 
+/* switch point of class q in the reference (see boys_switch) */
+static boys_switch reference_switch(const qclass *q) {
+  boys_switch sw;
+  const int c = (int)(q - CLASSES);
+  if (c <= 13 || (c >= 22 && c <= 25)) { /* rotated-axis and ERIC kernels */
+    sw.t = 25.0;
+    sw.ge = 0;
+  } else {
+    static const double thr[8] = {0, 0, 37.0, 43.0, 49.0, 55.0, 60.0, 65.0};
+    const int L = pair_l[q->bra][0] + pair_l[q->bra][1] + pair_l[q->ket][0] + pair_l[q->ket][1];
+    sw.t = thr[L / 2 + 1];
+    sw.ge = 1;
+  }
+  return sw;
+}
+
 int64_t oracle_fock_classes(const oracle_basis *b, const double *Dm, double *F, double cut,
                             int64_t counts[ORACLE_NCLASS], int64_t stride, int nthreads, const int32_t *enable) {
+  return oracle_fock_classes_sw(b, Dm, F, cut, counts, stride, nthreads, enable, 0);
+}
+
+int64_t oracle_fock_classes_sw(const oracle_basis *b, const double *Dm, double *F, double cut,
+                               int64_t counts[ORACLE_NCLASS], int64_t stride, int nthreads, const int32_t *enable,
+                               int ref_switch) {
   for (int s = 0; s < b->nsh; ++s)
     if (b->ang_mom[s] < 1 || b->ang_mom[s] > 4) return -1;
   if (stride < 1) stride = 1;
@@ -593,6 +648,7 @@ int64_t oracle_fock_classes(const oracle_basis *b, const double *Dm, double *F, 
       }
     }
     if (counts) counts[cidx] = n;
+    const boys_switch sw = reference_switch(q);
     int64_t nev = 0;
 #pragma omp parallel reduction(+ : nev)
     {
@@ -606,7 +662,7 @@ int64_t oracle_fock_classes(const oracle_basis *b, const double *Dm, double *F, 
         get_shell(b, sb, &B);
         get_shell(b, sc, &C);
         get_shell(b, sd, &D);
-        md_quartet(&A, &B, &C, &D, blk);
+        md_quartet(&A, &B, &C, &D, blk, ref_switch ? &sw : NULL);
         digest(&A, &B, &C, &D, sa == sb, sc == sd, diag && qb[t] == qk[t], blk, Dm, Fp, cut);
         ++nev;
       }

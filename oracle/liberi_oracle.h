@@ -42,6 +42,10 @@ typedef struct {
 int oracle_nclass(void);
 /* kernel name of class c in dispatch order of src/liberi_driver.F90:69-158 */
 const char *oracle_class_name(int c);
+/* class c's bra and ket pair classes (0..9, shell_pair_container_t order),
+ * quartet test (0: keep if >, 1: skip if <, 2: none) and pre-screen
+ * normalisation (0: the other list's maximum, 1: its own); -1 if c is out of range */
+int oracle_class_def(int c, int def[4]);
 
 /* Per-class surviving contracted shell-quartet counts (pure screening). */
 int oracle_screen_counts(const oracle_basis *b, double cutoff, int64_t counts[ORACLE_NCLASS]);
@@ -58,6 +62,14 @@ int64_t oracle_fock(const oracle_basis *b, const double *D, double *F, double cu
 int64_t oracle_fock_classes(const oracle_basis *b, const double *D, double *F, double cutoff,
                             int64_t counts[ORACLE_NCLASS], int64_t sample_stride, int nthreads,
                             const int32_t *enable);
+
+/* Same; ref_switch != 0 takes the reference's large-argument Boys values
+ * F_m = (2m-1)!!/(2T)^m sqrt(pi/4T) past each class's switch point (T > 25
+ * for its rotated-axis and ERIC kernels, X >= 37..65 by root count for its
+ * Rys kernels) instead of the exact ones. */
+int64_t oracle_fock_classes_sw(const oracle_basis *b, const double *D, double *F, double cutoff,
+                               int64_t counts[ORACLE_NCLASS], int64_t sample_stride, int nthreads,
+                               const int32_t *enable, int ref_switch);
 
 /* Schwarz integrals sqrt(max_{a in i, b in j} |(ab|ab)|), packed by shell pair
  * (index i(i-1)/2 + j, 1-based, i >= j), with the kernels' component scaling. */

@@ -23,6 +23,8 @@ expected outputs) is written under tests/golden/.
 from __future__ import annotations
 
 import argparse
+import json
+import math
 import os
 import re
 import subprocess
@@ -196,12 +198,287 @@ def fragment_fixture():
     np.savez_compressed(GOLD / "frag_c12_ref_classes.npz", **out)
 
 
+# ---------------------------------------------------------------- per-class parity
+DEFECT_TOL = 1e-10  # a class whose reference output departs from the oracle's by more
+
+
+def _oracle_class(orc, case, name, k, ref_switch):
+    """The oracle's Fock matrix of class k (cached under oracle/_ref/work)."""
+    f = OUT / "work" / ("oracle_sw_classes" if ref_switch else "oracle_classes") / name / f"{k}.f64"
+    if f.exists():
+        return np.fromfile(f, "<f8")
+    v, _, _ = orc.fock(case.basis, case.density, classes=[k], ref_switch=ref_switch)
+    f.parent.mkdir(parents=True, exist_ok=True)
+    v.tofile(f)
+    return v
+
+
+def class_fixture(name: str, candidates=None, nsample: int = 2048, full_below: int = 20000, seed: int = 5):
+    """The reference build's Fock matrix class by class (oracle/ref_classes.f90)
+    on workload `name` (tests/golden/cfg_<name>.npz), against the oracle with
+    the reference's large-argument switch (exact integrals up to each class's
+    switch point, the reference's asymptotic Boys values past it).  A class
+    whose reference output is off that oracle's by more than DEFECT_TOL is a
+    reference defect class (its tabulated Boys values): its full reference
+    vector is kept, with the oracle's (switch and exact) as float32
+    differences.  Of every other class a sample is kept: the nsample largest
+    reference values, the nsample elements where reference and exact oracle
+    differ most (where that oracle run exists), and nsample random nonzero
+    elements (the whole vector when ntri <= full_below).  `candidates`: the
+    classes to run the oracle on (None: all; on C60 the oracle takes hours);
+    the others are then taken as clean, which the parity test checks at the
+    1e-10 bar."""
+    from tests.oracle_ctypes import Oracle
+
+    orc = Oracle()
+    case = Case.load_npz(GOLD / f"cfg_{name}.npz")
+    counts = np.load(GOLD / f"cfg_{name}_counts.npy")
+    tmp = OUT / "work" / "classes" / name
+    tmp.mkdir(parents=True, exist_ok=True)
+    cf = tmp / "case.bin"
+    case.write_stream(cf)
+    rng = np.random.default_rng(seed)
+    ntri = case.basis.ntri
+    out, defect, summary = {}, [], {}
+    for c, k in enumerate(orc.class_names):
+        if counts[c] == 0:
+            continue
+        ff = tmp / f"{k}.f64"
+        if not ff.exists():
+            run([str(OUT / "liberi_ref_class"), str(cf), str(ff), k], env=REF_ENV, stdout=subprocess.DEVNULL)
+        ref = np.fromfile(ff, "<f8")
+        run_oracle = candidates is None or k in candidates
+        if run_oracle:
+            sw = _oracle_class(orc, case, name, k, True)
+            dev = float(np.max(np.abs(sw - ref)))
+            summary[k] = dev
+            if dev > DEFECT_TOL:
+                ex = _oracle_class(orc, case, name, k, False)
+                defect.append(k)
+                out[f"full_{k}"] = ref
+                out[f"oracle_sw_minus_ref_{k}"] = (sw - ref).astype(np.float32)
+                out[f"oracle_minus_ref_{k}"] = (ex - ref).astype(np.float32)
+                print(f"{name} {k}: reference defect, |reference - oracle(switch)| {dev:.2e}", flush=True)
+                continue
+        ef = OUT / "work" / "oracle_classes" / name / f"{k}.f64"
+        d = np.fromfile(ef, "<f8") - ref if ef.exists() else None
+        if ntri <= full_below:
+            idx = np.arange(ntri)
+        else:
+            nz = np.nonzero(ref)[0]
+            idx = np.argsort(-np.abs(ref))[:nsample]
+            if d is not None:
+                idx = np.union1d(idx, np.argsort(-np.abs(d))[:nsample])
+            idx = np.union1d(idx, rng.choice(nz, min(nsample, nz.size), replace=False))
+        out[f"idx_{k}"] = idx.astype(np.int32)
+        out[f"ref_{k}"] = ref[idx]
+        print(f"{name} {k}: |reference - oracle(switch)| {summary.get(k, float('nan')):.2e}", flush=True)
+    out["defect"] = np.array(defect)
+    out["source"] = np.array("gms_libERI CPU path built with AMD flang 22 (oracle/Makefile ref), one class per run "
+                             "(oracle/ref_classes.f90); oracle: oracle/liberi_oracle.c (exact and with the "
+                             "reference's large-argument switch); tests/golden/make_reference_fixtures.py:class_fixture")
+    np.savez_compressed(GOLD / f"cfg_{name}_ref_classes.npz", **out)
+    print(f"{name}: reference defect classes (|reference - oracle(switch)| > {DEFECT_TOL:g}): "
+          f"{ {k: summary[k] for k in defect} }")
+
+
+# ---------------------------------------------------------------- screening edges
+CUT = 5.00000000000000018e-11  # cutoff_schwarz, src/liberi_parameters.F90:38
+PAIR_CLASSES = [(0, 0), (0, 1), (1, 1), (0, 2), (1, 2), (2, 2), (0, 3), (1, 3), (2, 3), (3, 3)]
+EDGE_MAX = [0.5, 1.0, 2.0, 4.0]
+# two colourings of the 10 pair classes by list maximum: every two pair classes
+# get different maxima in at least one of the two fixtures, so that for every
+# non-diagonal class one fixture tells the two pre-screen normalisations apart
+EDGE_FIXTURES = {"screen_edges_a": lambda k: k % 4, "screen_edges_b": lambda k: k // 4}
+
+
+def _pair_class(la, lb):
+    return PAIR_CLASSES.index((min(la, lb), max(la, lb)))
+
+
+def _edge_values(rng, n, vmax):
+    """n crafted Schwarz values of one pair class, the largest exactly vmax:
+    the thresholds cut/M of every list maximum M in use (exact in binary, so
+    their products with a list maximum hit the cutoff exactly), this list's
+    own threshold one ulp below, then values drawn log-uniformly over
+    [cut/16, 16 cut] (between the thresholds of the two normalisations), one
+    ulp below another list's threshold, or large."""
+    c = CUT
+    first = [vmax] + [c / m for m in EDGE_MAX] + [np.nextafter(c / vmax, 0.0)]
+    rest = []
+    for _ in range(max(0, n - len(first))):
+        r = rng.random()
+        if r < 0.5:
+            rest.append(float(np.exp(rng.uniform(np.log(c / 16), np.log(16 * c)))))
+        elif r < 0.8:
+            rest.append(float(np.nextafter(c / rng.choice(EDGE_MAX), 0.0)))
+        else:
+            rest.append(float(rng.choice([v for v in (0.25, 0.5, 1.0, 1e-3) if v <= vmax])))
+    vals = (first + rest)[:n]
+    rng.shuffle(vals)
+    return np.array(vals)
+
+
+def screening_edges_case(name: str, seed: int):
+    """Three atoms carrying s, s, p, p, d, d and f shells (every shell pair
+    class) and a seeded density; Schwarz values crafted per pair class (not
+    bounds of the integrals) so that the pre-screen and the quartet test meet
+    their thresholds exactly and the two pre-screen normalisations keep
+    different pairs.  The list maximum of pair class k is
+    EDGE_MAX[EDGE_FIXTURES[name](k)]."""
+    from gms_liberi_amd import configs
+
+    shells = [(0, 2, "wide"), (0, 1, "wide"), (1, 1, "wide"), (1, 1, "wide"), (2, 1, "d"), (2, 1, "d"),
+              (3, 1, "f")]
+    bond, ang = 1.45, math.radians(112.0)
+    xyz = [(0.0, 0.0, 0.0), (bond, 0.0, 0.0), (bond - bond * math.cos(ang), bond * math.sin(ang), 0.3)]
+    case = configs.build_case(name, ["C", "N", "C"], xyz, lambda s: shells, 16, seed)
+    b = case.basis
+    rng = np.random.default_rng(seed + 1)
+    l = b.ang_mom - 1
+    pc = {}
+    for i in range(b.nsh):
+        for j in range(i + 1):
+            pc.setdefault(_pair_class(l[i], l[j]), []).append(i * (i + 1) // 2 + j)
+    x = np.zeros(b.nsh * (b.nsh + 1) // 2)
+    colour = EDGE_FIXTURES[name]
+    for k, idx in sorted(pc.items()):
+        x[idx] = _edge_values(rng, len(idx), EDGE_MAX[colour(k)])
+    case.basis = b.with_schwarz(x)
+    return case, pc
+
+
+def _survivors(xb, xk, diag, ps, cmp_):
+    """(bra, ket) positions of the surviving quartets under a pre-screen
+    normalisation and quartet test (restated for the coverage check only)."""
+    tb, tk = (CUT / xk.max(), CUT / xb.max()) if ps == "cross" else (CUT / xb.max(), CUT / xk.max())
+    kb, kk = np.nonzero(xb >= tb)[0], np.nonzero(xk >= tk)[0]
+    out = set()
+    for i in kb:
+        for j in kk:
+            if diag and j > i:
+                continue
+            p = xb[i] * xk[j]
+            if cmp_ == "none" or (cmp_ == "gt" and p > CUT) or (cmp_ == "ge" and p >= CUT):
+                out.add((int(i), int(j)))
+    return out, tb, tk
+
+
+def screening_edge_coverage(case, pc, class_defs):
+    """Per class: pairs exactly at their pre-screen threshold and one ulp below
+    it, quartets of the pre-screened lists whose product equals the cutoff
+    exactly, and whether the other pre-screen normalisation (and, where the
+    class has a test, the other comparator) would keep a different set."""
+    x = case.basis.schwrz
+    out = {}
+    for name, bra, ket, cmp_, ps in class_defs:
+        xb, xk = x[pc[bra]], x[pc[ket]]
+        diag = bra == ket
+        keep, tb, tk = _survivors(xb, xk, diag, ps, cmp_)
+        other_ps, _, _ = _survivors(xb, xk, diag, "self" if ps == "cross" else "cross", cmp_)
+        at = int(np.sum(xb == tb) + (0 if diag else np.sum(xk == tk)))
+        below = int(np.sum(xb == np.nextafter(tb, 0.0)) + (0 if diag else np.sum(xk == np.nextafter(tk, 0.0))))
+        ties = sum(1 for i in range(len(xb)) for j in range(len(xk)) if (not diag or j <= i)
+                   and xb[i] >= tb and xk[j] >= tk and xb[i] * xk[j] == CUT)
+        c = {"test": cmp_, "prescreen": ps, "survivors": len(keep), "at_threshold": at, "below_threshold": below,
+             "tie_quartets": ties, "other_normalisation_differs": keep != other_ps}
+        if cmp_ != "none":
+            other_cmp, _, _ = _survivors(xb, xk, diag, ps, "ge" if cmp_ == "gt" else "gt")
+            c["other_comparator_differs"] = keep != other_cmp
+        out[name] = c
+    return out
+
+
+def _covered(cov, defs, colour):
+    for name, bra, ket, cmp_, ps in defs:
+        c = cov[name]
+        if not (c["at_threshold"] and c["below_threshold"] and c["tie_quartets"]):
+            return False
+        if cmp_ != "none" and not c["other_comparator_differs"]:
+            return False
+        # the normalisations differ only between lists of different maxima
+        # (for a diagonal class both are the same list's maximum)
+        if colour(bra) != colour(ket) and not c["other_normalisation_differs"]:
+            return False
+    return True
+
+
+def _sensitivity(orc, case):
+    """Oracle-measured size of a flipped edge, per class: max |dF| when the
+    cutoff moves one ulp up (ties and threshold pairs dropped) or down (pairs
+    and quartets one ulp short kept)."""
+    res = {}
+    up, dn = np.nextafter(CUT, 1.0), np.nextafter(CUT, 0.0)
+    for k in orc.class_names:
+        f0, _, _ = orc.fock(case.basis, case.density, classes=[k])
+        fu, _, _ = orc.fock(case.basis, case.density, classes=[k], cutoff=up)
+        fd, _, _ = orc.fock(case.basis, case.density, classes=[k], cutoff=dn)
+        res[k] = (float(np.max(np.abs(fu - f0))), float(np.max(np.abs(fd - f0))))
+    return res
+
+
+def screening_edges_fixture(seed0: int = 21, min_effect: float = 1e-7):
+    """Reference build, class by class, on the two crafted screening cases:
+    for each, the first seed from seed0 on whose Schwarz values cover every
+    edge of every class, and where the oracle moves each class's Fock matrix
+    by at least min_effect when the cutoff moves one ulp in a direction that
+    matters for that class's comparator."""
+    from tests.oracle_ctypes import Oracle
+
+    orc = Oracle()
+    defs = orc.class_defs
+    summary = {}
+    for name, colour in EDGE_FIXTURES.items():
+        for seed in range(seed0, seed0 + 400):
+            case, pc = screening_edges_case(name, seed)
+            cov = screening_edge_coverage(case, pc, defs)
+            if not _covered(cov, defs, colour):
+                continue
+            sens = _sensitivity(orc, case)
+            weak = []
+            for cname, bra, ket, cmp_, ps in defs:
+                up, dn = sens[cname]
+                need = {"ge": (up, dn), "gt": (dn,), "none": (up, dn)}[cmp_]
+                if min(need) < min_effect:
+                    weak.append(cname)
+                cov[cname]["effect_cut_up"], cov[cname]["effect_cut_down"] = up, dn
+            print(f"{name} seed {seed}: every edge covered; classes below {min_effect:g}: {weak}", flush=True)
+            if not weak:
+                break
+        else:
+            raise SystemExit(f"{name}: no seed covers every screening edge")
+        tmp = OUT / "work"
+        tmp.mkdir(parents=True, exist_ok=True)
+        cf, ff = tmp / f"{name}.case", tmp / f"{name}.fock"
+        case.write_stream(cf)
+        per = {}
+        for k in orc.class_names:
+            run([str(OUT / "liberi_ref_class"), str(cf), str(ff), k], env=REF_ENV, stdout=subprocess.DEVNULL)
+            per[k] = np.fromfile(ff, "<f8")
+        fock, _ = reference_fock(case)
+        print(f"{name} (seed {seed}): nbf={case.basis.num_bas}, sum of classes vs full build "
+              f"{np.max(np.abs(sum(per.values()) - fock)):.2e}")
+        case.fock = fock
+        case.source = ("gms_libERI CPU path built with AMD flang 22 (oracle/Makefile ref), full build; per-class "
+                       f"builds (oracle/ref_classes.f90) in {name}_ref_classes.npz; inputs and crafted Schwarz "
+                       f"values from tests/golden/make_reference_fixtures.py:screening_edges_case({name!r}, "
+                       f"seed={seed})")
+        case.save_npz(GOLD / f"{name}.npz")
+        np.savez_compressed(GOLD / f"{name}_ref_classes.npz", **per)
+        summary[name] = {"seed": seed, "maxima": [EDGE_MAX[colour(k)] for k in range(10)], "classes": cov}
+    (GOLD / "screen_edges_coverage.json").write_text(json.dumps(summary, indent=1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-build", action="store_true")
     ap.add_argument("--configs", nargs="*", default=["water", "ethane"])
     ap.add_argument("--no-ref-tests", action="store_true")
     ap.add_argument("--fragment", action="store_true")
+    ap.add_argument("--screening-edges", action="store_true")
+    ap.add_argument("--class-fixtures", nargs="*", default=[], help="per-class reference fixtures of these workloads")
+    ap.add_argument("--oracle-classes", nargs="*", default=None,
+                    help="classes to run the oracle on for --class-fixtures (default: all)")
     a = ap.parse_args()
     if not a.skip_build:
         build_reference()
@@ -211,6 +488,10 @@ def main():
         config_fixtures(a.configs)
     if a.fragment:
         fragment_fixture()
+    if a.screening_edges:
+        screening_edges_fixture()
+    for name in a.class_fixtures:
+        class_fixture(name, a.oracle_classes)
 
 
 if __name__ == "__main__":

@@ -36,12 +36,21 @@ class Oracle:
         L.oracle_fock.restype = C.c_int64
         L.oracle_fock_classes.argtypes = L.oracle_fock.argtypes + [C.POINTER(C.c_int32)]
         L.oracle_fock_classes.restype = C.c_int64
+        L.oracle_fock_classes_sw.argtypes = L.oracle_fock_classes.argtypes + [C.c_int]
+        L.oracle_fock_classes_sw.restype = C.c_int64
         L.oracle_schwarz.argtypes = [C.POINTER(_OBasis), C.POINTER(C.c_double)]
         L.oracle_shell_quartet.argtypes = [C.POINTER(_OBasis), C.c_int, C.c_int, C.c_int, C.c_int,
                                            C.POINTER(C.c_double)]
         L.oracle_boys.argtypes = [C.c_int, C.c_double]
         L.oracle_boys.restype = C.c_double
         self.class_names = [L.oracle_class_name(i).decode() for i in range(NCLASS)]
+        L.oracle_class_def.argtypes = [C.c_int, C.POINTER(C.c_int)]
+        self.class_defs = []  # (name, bra pair class, ket pair class, "gt"|"ge"|"none", "cross"|"self")
+        for i in range(NCLASS):
+            d = (C.c_int * 4)()
+            L.oracle_class_def(i, d)
+            self.class_defs.append((self.class_names[i], d[0], d[1], ("gt", "ge", "none")[d[2]],
+                                    ("cross", "self")[d[3]]))
 
     def _basis(self, b: Basis, schwrz=None):
         keep = []
@@ -62,7 +71,11 @@ class Oracle:
         self.lib.oracle_screen_counts(C.byref(ob), cutoff, _ptr(out, C.c_int64))
         return out
 
-    def fock(self, b: Basis, density, cutoff=5.00000000000000018e-11, stride=1, nthreads=0, classes=None):
+    def fock(self, b: Basis, density, cutoff=5.00000000000000018e-11, stride=1, nthreads=0, classes=None,
+             ref_switch=False):
+        """Packed Fock matrix (exact integrals, or with ref_switch the
+        reference's large-argument Boys values past its switch points), the
+        per-class quartet counts and the number of quartets evaluated."""
         ob, keep = self._basis(b)
         d = np.ascontiguousarray(density, np.float64)
         f = np.zeros(b.ntri)
@@ -71,9 +84,9 @@ class Oracle:
         if classes is not None:
             en = np.array([1 if (n in classes or i in classes) else 0 for i, n in enumerate(self.class_names)],
                           np.int32)
-        n = self.lib.oracle_fock_classes(C.byref(ob), _ptr(d, C.c_double), _ptr(f, C.c_double), cutoff,
-                                         _ptr(counts, C.c_int64), stride, nthreads,
-                                         None if en is None else _ptr(en, C.c_int32))
+        n = self.lib.oracle_fock_classes_sw(C.byref(ob), _ptr(d, C.c_double), _ptr(f, C.c_double), cutoff,
+                                            _ptr(counts, C.c_int64), stride, nthreads,
+                                            None if en is None else _ptr(en, C.c_int32), int(ref_switch))
         if n < 0:
             raise RuntimeError("oracle_fock failed")
         return f, counts, n

@@ -3,7 +3,9 @@ tau are evaluated in FP32 (Rys roots/weights and Fock accumulation in FP64).
 
 Parity bar: the energy error against the FP64 build stays within the bound
 the switch implies and within 1e-6 Eh at tau = 1e-6; tau = 0 is the plain
-FP64 build.
+FP64 build.  With the autotuner off every lane-per-quartet class takes the
+FP32 path (the error tests); with it on, each class keeps FP32 only where it
+is measured faster.
 """
 from pathlib import Path
 
@@ -62,16 +64,54 @@ def test_mixed_precision_error(ethane, tau):
     print(f"tau {tau:g}: fp32 quartets {nf32}/{total}, max|dF| {dF:.2e}, |dE| {dE:.2e}")
 
 
-@pytest.mark.gpu
-def test_mixed_precision_c60_energy():
+@pytest.fixture(scope="module")
+def c60_fp64():
     case = Case.load_npz(GOLD / "cfg_c60.npz")
     h = liberi.Handle()
     h.setup(case.basis)
     ref = h.fock_build(case.density)
-    h.set_option(liberi.OPT_FP32_TAU, 1e-6)
+    h.destroy()
+    return case, ref
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tau", [1e-4, 1e-5, 1e-6, 1e-7])
+def test_mixed_precision_c60_sweep(c60_fp64, tau):
+    """The C60 cage with FP32 forced into every lane-per-quartet class:
+    max|dF| and dE against FP64 per switch tau."""
+    case, ref = c60_fp64
+    h = liberi.Handle()
+    h.set_option(liberi.OPT_AUTOTUNE, 0)
+    h.set_option(liberi.OPT_FP32_TAU, tau)
+    h.setup(case.basis)
     mix = h.fock_build(case.density)
-    nf32 = h.stats()["fp32_quartets"]
+    st = h.stats()
     h.destroy()
     dE = abs(_energy(case, mix) - _energy(case, ref))
-    assert nf32 > 0
-    assert dE <= ENERGY_TOL_TAU6
+    dF = float(np.max(np.abs(mix - ref)))
+    print(f"C60 tau {tau:g}: fp32 quartets {st['fp32_quartets']}/{st['total_quartets']}, max|dF| {dF:.2e}, "
+          f"|dE| {dE:.2e}")
+    assert st["fp32_quartets"] > 0
+    assert dF <= tau
+    if tau <= 1e-6:
+        assert dE <= ENERGY_TOL_TAU6
+
+
+@pytest.mark.gpu
+def test_autotuned_mixed_form_chooses_per_class(c60_fp64):
+    """With the autotuner on, FP32 stays only in lane-per-quartet classes
+    where it was measured faster; the FP32 count matches that choice and the
+    error stays within the forced form's bound."""
+    case, ref = c60_fp64
+    h = liberi.Handle()
+    h.setup(case.basis)
+    h.set_option(liberi.OPT_FP32_TAU, 1e-6)
+    mix = h.fock_build(case.density)
+    st = h.stats()
+    h.destroy()
+    chosen = [liberi.class_names()[c] for c in range(55) if st["class_fp32"][c]]
+    print(f"FP32 kept in {chosen}; fp32 quartets {st['fp32_quartets']}")
+    if not chosen:
+        assert st["fp32_quartets"] == 0
+    dE = abs(_energy(case, mix) - _energy(case, ref))
+    assert float(np.max(np.abs(mix - ref))) <= 1e-6 and dE <= ENERGY_TOL_TAU6

@@ -3,12 +3,18 @@
 Each request's Fock matrices must be the same run alone and in the mix
 (concurrent streams only change the order of the atomic additions: within
 1e-11 relative to the largest element), in FP64 and in the mixed form, and
-equal to a plain handle build of the same molecule.
+equal to a plain handle build of the same molecule; a served request's Fock
+matrix equals the reference build's on the same molecule and density.
 """
+from pathlib import Path
+
 import numpy as np
 import pytest
 
 from gms_liberi_amd import liberi, serving
+from gms_liberi_amd.caseio import Case
+
+GOLD = Path(__file__).resolve().parent / "golden"
 
 MIX_TOL = 1e-11
 
@@ -74,3 +80,57 @@ def test_footprint_estimate_covers_the_handle():
     for r in _stream():
         res = serving.Server(max_active=1).run([r])[0]
         assert serving.estimate_device_bytes(r.basis) >= res.device_bytes
+
+
+@pytest.mark.gpu
+def test_full_size_stream_alone_equals_mix():
+    """The stream recipe at its full range: a cluster of 150-200 atoms among
+    smaller ones, 10 SCF-like iterations each, arriving together."""
+    big = serving.request_stream(1, rate=1.0, seed=33, iterations=10, atoms=(150, 200))
+    small = serving.request_stream(3, rate=1.0, seed=34, iterations=10, atoms=(10, 60))
+    reqs = [serving.Request(i, 0.0, r.basis, r.densities) for i, r in enumerate(big + small)]
+    natoms = [r.basis.natoms for r in reqs]
+    print(f"atoms per request {natoms}, nbf {[r.basis.num_bas for r in reqs]}", flush=True)
+    assert 150 <= natoms[0] <= 200
+    alone = []
+    for r in reqs:
+        alone += serving.Server(max_active=1).run([r])
+    srv = serving.Server(max_active=4)
+    mix = srv.run(reqs)
+    assert srv.peak_active >= 2
+    for a, m in zip(alone, mix):
+        assert a.rid == m.rid and len(a.focks) == len(m.focks) == 10
+        for fa, fm in zip(a.focks, m.focks):
+            assert _close(fa, fm) <= MIX_TOL
+
+
+@pytest.mark.gpu
+def test_served_fock_matches_reference_build():
+    """A 12-atom request of the stream's recipe (tests/golden/cfg_cluster12.npz,
+    Fock matrix from the reference build) served among other requests: its
+    first iteration's Fock matrix equals a plain handle build's and the
+    reference's.  Against the reference the bound is 1e-10 plus, for each
+    class where the reference's tabulated Boys values put it off the oracle
+    by more than that (cfg_cluster12_ref_classes.npz, checked class by class
+    in test_gpu_parity.py), that departure plus 1e-11.  The served handle takes
+    Schwarz integrals from the device, the fixture from the oracle's engine;
+    the surviving quartets are the same."""
+    case = Case.load_npz(GOLD / "cfg_cluster12.npz")
+    cls = np.load(GOLD / "cfg_cluster12_ref_classes.npz")
+    other = serving.request_stream(2, rate=1.0, seed=35, iterations=3, atoms=(20, 40))
+    dens = [case.density] + serving.scf_like_densities(case.basis.num_bas, 20, 2, seed=5)
+    reqs = [serving.Request(0, 0.0, case.basis, dens)] + [serving.Request(i + 1, 0.0, r.basis, r.densities)
+                                                          for i, r in enumerate(other)]
+    srv = serving.Server(max_active=3)
+    res = srv.run(reqs)
+    assert srv.peak_active >= 2
+    f = res[0].focks[0]
+    h = liberi.Handle()
+    h.setup(case.basis)
+    assert _close(h.fock_build(case.density), f) <= MIX_TOL
+    h.destroy()
+    bound = 1e-10 + sum(float(np.max(np.abs(cls[f"oracle_sw_minus_ref_{k}"]))) + 1e-11 for k in cls["defect"])
+    err = float(np.max(np.abs(f - case.fock)))
+    print(f"served vs reference: {err:.2e} (bound {bound:.2e}; reference defect classes {list(cls['defect'])})")
+    assert err <= bound
+    assert res[0].quartets == int(np.load(GOLD / "cfg_cluster12_counts.npy").sum())
