@@ -63,6 +63,20 @@ def profiled_traffic(workload: str, cls: str, variant: int):
     return None
 
 
+def profiled_counter_tflops(workload: str, cls: str):
+    """FP64 FLOP/s of the class's kernel from the committed rocprof
+    instruction-counter summary (64 lanes per FP64 instruction, FMA = 2), or
+    None when it was not profiled."""
+    f = ROOT / "profiles" / "fock_build_c60_f64_counters.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        p = [x.strip() for x in line.split("|")]
+        if len(p) > 6 and p[1] == workload and p[3] == cls:
+            return {"counter_tflops": float(p[5]), "kernel": p[2].strip("`"), "source": f"profiles/{f.name}"}
+    return None
+
+
 def timed_builds(h, D, F, n, stream, steps, hipmem):
     start, end = hipmem.Event(), hipmem.Event()
     stream.synchronize()
@@ -153,8 +167,9 @@ def gpu_leg(case, steps: int, warmup: int, taus=(), graph: bool = True, host_ent
     achieved = flops[dom] / (class_ms[dom] * 1e-3)
     total_flops = float(flops.sum())
     per_class = {census.NAMES[c]: {"ms": round(float(class_ms[c]), 4), "quartets": int(q[c]),
-                                   "tflops": round(flops[c] / max(class_ms[c], 1e-9) * 1e-9, 3)}
-                 for c in np.argsort(-class_ms)[:8] if class_ms[c] > 0}
+                                   "tflops": round(flops[c] / max(class_ms[c], 1e-9) * 1e-9, 3),
+                                   "form": int(st["class_variant"][c])}
+                 for c in np.argsort(-class_ms) if class_ms[c] > 0}
     return dict(ms_per_step=ms_per_step, fock=fock, stats=st, dom=dom, achieved=achieved,
                 dom_ms=float(class_ms[dom]), dom_flops=float(flops[dom]), total_flops=total_flops,
                 per_class=per_class, setup_s=setup_s, mixed=mixed, host=host)
@@ -208,29 +223,34 @@ def cpu_leg(case, reps: int):
 
 
 def rank_shares(case, nprocs: int, steps: int):
-    """Each of `nprocs` rank shares (my_rank/num_procs, the reference's split
-    of every class's quartet range) built alone on this GPU: the time of each
-    share, the imbalance max/mean, and the sum of the partial Fock matrices
-    against the one-rank build."""
+    """Each of `nprocs` rank shares (my_rank/num_procs) built alone on this
+    GPU, with the default cost-balanced split and with the reference's equal
+    quartet counts: the time of each share, the imbalance max/mean, and the
+    sum of the partial Fock matrices against the reference."""
     from gms_liberi_amd import hipmem, liberi
 
     n = case.basis.ntri
     D = hipmem.DeviceArray.from_host(case.density)
     F = hipmem.DeviceArray(n)
     stream = hipmem.Stream()
-    ms, total = [], np.zeros(n)
-    for r in range(nprocs):
-        h = liberi.Handle()
-        h.setup(case.basis, r, nprocs)
-        h.fock_build_device(D.address, F.address, n, stream.address)  # autotuning
-        h.fock_build_device(D.address, F.address, n, stream.address)
-        ms.append(timed_builds(h, D, F, n, stream, steps, hipmem))
-        total += F.to_host()
-        h.destroy()
+    out = {"num_procs": nprocs}
+    for label, split in (("balanced", 0), ("equal_counts", 1)):
+        ms, total = [], np.zeros(n)
+        for r in range(nprocs):
+            h = liberi.Handle()
+            h.set_option(liberi.OPT_RANK_SPLIT, split)
+            h.setup(case.basis, r, nprocs)
+            h.fock_build_device(D.address, F.address, n, stream.address)  # autotuning
+            h.fock_build_device(D.address, F.address, n, stream.address)
+            ms.append(timed_builds(h, D, F, n, stream, steps, hipmem))
+            total += F.to_host()
+            h.destroy()
+        mean = float(np.mean(ms))
+        out[label] = dict(ms=[round(m, 3) for m in ms], max_over_mean=round(max(ms) / mean, 4),
+                          sum_ms=round(float(np.sum(ms)), 3),
+                          max_abs_dF_sum_vs_reference=float(np.max(np.abs(total - case.fock))))
     stream.destroy()
-    mean = float(np.mean(ms))
-    return dict(num_procs=nprocs, ms=[round(m, 3) for m in ms], max_over_mean=round(max(ms) / mean, 4),
-                sum_ms=round(float(np.sum(ms)), 3), max_abs_dF_sum_vs_reference=float(np.max(np.abs(total - case.fock))))
+    return out
 
 
 def serving_leg(n: int, rate: float, seed: int, tau: float, slots: int):
@@ -347,10 +367,11 @@ def main():
                     "total_tflops": round(ge["total_flops"] / (ge["ms_per_step"] * 1e-3) * 1e-12, 3),
                     "accuracy_vs_reference": {"max_abs_dF": float(np.max(np.abs(ge["fock"] - ce.fock))),
                                               "dE_2e": float(0.5 * np.dot(ce.density, ge["fock"] - ce.fock))},
-                    "top_kernels": ge["per_class"]}
+                    "top_kernels": dict(list(ge["per_class"].items())[:8])}
     shares = rank_shares(case, a.rank_shares, a.steps) if a.rank_shares > 1 else None
     if shares:
-        log(f"rank shares of {a.rank_shares}: {shares['ms']} ms, max/mean {shares['max_over_mean']}")
+        for k in ("balanced", "equal_counts"):
+            log(f"rank shares of {a.rank_shares} ({k}): {shares[k]['ms']} ms, max/mean {shares[k]['max_over_mean']}")
     serve = serving_leg(a.serve_requests, a.serve_rate, 5, a.tau, a.serve_slots) if a.serve_requests > 0 else None
     cpu = None if a.no_cpu else cpu_leg(case, a.cpu_reps)
     cpu_base = None
@@ -394,8 +415,10 @@ def main():
                      "frac": round(g["achieved"] / FP64_VALU_PEAK, 4),
                      "traffic": traffic["bytes"] if traffic else None,
                      "traffic_detail": traffic,
-                     "kernel_ms": round(g["dom_ms"], 4), "kernel_flops": g["dom_flops"]},
-        "top_kernels": g["per_class"],
+                     "kernel_ms": round(g["dom_ms"], 4), "kernel_flops": g["dom_flops"],
+                     "counters": profiled_counter_tflops(a.workload, census.NAMES[dom])},
+        "top_kernels": dict(list(g["per_class"].items())[:8]),
+        "classes": g["per_class"],
         "host_entry": None if not g["host"] else {
             "value": round(1000.0 / g["host"]["ms"], 4), "unit": "Fock builds/s", "ms_per_build": round(g["host"]["ms"], 4),
             "note": "liberi_c_fock_build on host arrays: D upload and F download included"},

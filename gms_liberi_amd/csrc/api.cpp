@@ -99,6 +99,7 @@ struct liberi_ctx {
   int row_tiles = 1;  // 0 never, 1 per class (autotuned or default), 2 wherever they fit
   bool autotune = true;
   double max_device_bytes = 0.0;  // HBM budget of the handle (0: none)
+  int rank_split = 0;             // 0 shares balanced by estimated cost, 1 equal quartet counts
   // basis
   bool initialized = false;
   int32_t nsh = 0, natoms = 0, num_bas = 0, mxgtot = 0, rank = 0, nprocs = 1;
@@ -315,6 +316,10 @@ int liberi_c_set_option(liberi_ctx *h, int32_t opt, double v) {
       break;
     case LIBERI_OPT_GRAPH: h->use_graph = v != 0.0; break;
     case LIBERI_OPT_MAX_DEVICE_BYTES: h->max_device_bytes = v; break;
+    case LIBERI_OPT_RANK_SPLIT:
+      if (v != 0.0 && v != 1.0) return fail(LIBERI_EARG, "liberi_c_set_option: RANK_SPLIT must be 0 or 1");
+      h->rank_split = (int)v;
+      break;
     case LIBERI_OPT_AUTOTUNE:
       h->autotune = v != 0.0;
       // decide (and time) again at the next build, without earlier handles' choices
@@ -340,6 +345,7 @@ int liberi_c_get_option(liberi_ctx *h, int32_t opt, double *v) {
     case LIBERI_OPT_AUTOTUNE: *v = h->autotune; break;
     case LIBERI_OPT_GRAPH: *v = h->use_graph; break;
     case LIBERI_OPT_MAX_DEVICE_BYTES: *v = h->max_device_bytes; break;
+    case LIBERI_OPT_RANK_SPLIT: *v = h->rank_split; break;
     default: return fail(LIBERI_EARG, "liberi_c_get_option: unknown option " + std::to_string(opt));
   }
   return LIBERI_OK;
@@ -733,10 +739,46 @@ static int setup_basis(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_b
     d.seg_ket0 = (const int32_t *)cs_.seg_ket0.p;
     d.nseg = cs_.nseg;
     d.diag = diag;
-    // this rank's contiguous share of the class (the reference splits each
-    // class's index range the same way, e.g. src/rhf/rot_axis/int0001.F90:100-104)
-    d.q_begin = total * my_rank / num_procs;
-    d.q_end = total * (my_rank + 1) / num_procs;
+    // this rank's contiguous share of the class.  The reference splits each
+    // class's index range into equal counts (e.g.
+    // src/rhf/rot_axis/int0001.F90:100-104; LIBERI_OPT_RANK_SPLIT = 1);
+    // by default the shares are equal in estimated cost instead: a quartet
+    // costs its primitive quartets plus kQuartetCost for its HRR and
+    // digestion, and the kept lists run from few to many primitives, so equal
+    // counts leave the last ranks the most work (C60 in 8 shares, time of the
+    // slowest over the mean: equal counts 1.39; kQuartetCost 4: 1.26,
+    // 8: 1.08, 16: 1.08, 32: 1.16).
+    if (h->rank_split == 1 || num_procs == 1) {
+      d.q_begin = total * my_rank / num_procs;
+      d.q_end = total * (my_rank + 1) / num_procs;
+    } else {
+      constexpr double kQuartetCost = 12.0;
+      const size_t ns = sbra.size();
+      std::vector<double> cum(ns + 1, 0.0), unit(ns);
+      for (size_t k = 0; k < ns; ++k) {
+        unit[k] = (double)pairs[sbra[k]].nprim * pairs[kk[sket0[k]]].nprim + kQuartetCost;
+        cum[k + 1] = cum[k] + unit[k] * (double)(sstart[k + 1] - sstart[k]);
+      }
+      // first quartet at or past cost b
+      auto q_at = [&](double b) -> int64_t {
+        if (b <= 0.0) return 0;
+        if (b >= cum[ns]) return total;
+        const size_t k = (size_t)(std::upper_bound(cum.begin(), cum.end(), b) - cum.begin()) - 1;
+        const int64_t q = sstart[k] + (int64_t)std::ceil((b - cum[k]) / unit[k]);
+        return std::min(q, sstart[k + 1]);
+      };
+      d.q_begin = q_at(cum[ns] * my_rank / num_procs);
+      d.q_end = my_rank + 1 == num_procs ? total : q_at(cum[ns] * (my_rank + 1) / num_procs);
+    }
+    // this share's primitive quartets
+    {
+      int64_t pq = 0;
+      for (size_t k = 0; k < sbra.size(); ++k) {
+        const int64_t a = std::max(sstart[k], d.q_begin), b = std::min(sstart[k + 1], d.q_end);
+        if (b > a) pq += (b - a) * (int64_t)pairs[sbra[k]].nprim * pairs[kk[sket0[k]]].nprim;
+      }
+      primq = pq;
+    }
     // tiles overlapping [q_begin, q_end): tq is strictly increasing, tq.back() = total
     const int32_t ntile = (int32_t)tq.size() - 1;
     d.tile_q = (const int64_t *)cs_.tile_q.p;
@@ -802,15 +844,16 @@ constexpr int kSideStreams = 4;
 
 // Choices made so far in this process: later handles (a request loop sets up
 // one per molecule) reuse them instead of timing again, but only for a class
-// of about the same size -- the key is the class and the power of two of its
-// quartet count -- and, for the FP32 choice, the same switch tau.
+// of about the same size -- the key is the class and its quartet count to
+// within a factor of 8 -- and, for the FP32 choice, the same switch tau.
 static std::mutex g_tune_mu;
 static std::map<std::tuple<int, int, double>, int> g_tuned;  // (class, size, tau or -1) -> choice
 
+// classes within a factor of 8 in quartet count share a choice
 static int size_bucket(int64_t nq) {
   int b = 0;
   while (nq > 1) nq >>= 1, ++b;
-  return b;
+  return b / 3;
 }
 
 static int tuned_choice(int c, int64_t nq, double tau) {

@@ -153,6 +153,11 @@ const char *liberi_c_last_error(void);
                                        the same D and F replay; 0: one stream, launched eagerly.
                                        Per-class timing and ONLY_CLASS builds run eagerly.  Read at
                                        each build; any option change drops the captured graph. */
+#define LIBERI_OPT_RANK_SPLIT 12     /* rank shares of every class (my_rank/num_procs of setup): 0 (default)
+                                        contiguous shares of equal estimated cost (primitive quartets plus
+                                        a per-quartet term); 1 equal quartet counts, the reference's split
+                                        (e.g. src/rhf/rot_axis/int0001.F90:100-104).  Either way the ranks'
+                                        partial Fock matrices sum to the full one.  Read at setup. */
 #define LIBERI_OPT_MAX_DEVICE_BYTES 11 /* HBM budget of the handle in bytes (0, the default: none).  An
                                           allocation that would take the handle's device data past it
                                           fails with LIBERI_ENOMEM; a setup that fails that way (or any

@@ -161,22 +161,31 @@ def test_reference_named_functions():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("nprocs", [2, 3])
-def test_rank_partials_sum_to_full(nprocs):
-    """Each rank returns its partial F; the caller's sum is the full F."""
+@pytest.mark.parametrize("split", [0, 1])
+@pytest.mark.parametrize("nprocs", [2, 3, 8])
+def test_rank_partials_sum_to_full(nprocs, split):
+    """Each rank returns its partial F; the caller's sum is the full F, for the
+    cost-balanced shares (split 0) and the reference's equal counts (split 1);
+    the shares' quartets and primitive quartets add up to the class totals."""
     case = Case.load_npz(GOLD / "cfg_ethane.npz")
     h = liberi.Handle()
+    h.set_option(liberi.OPT_RANK_SPLIT, split)
     h.setup(case.basis)
     full = h.fock_build(case.density)
     total_q = h.stats()["quartets"].copy()
+    total_pq = h.stats()["prim_quartets"].copy()
     acc = np.zeros_like(full)
     q = np.zeros_like(total_q)
+    pq = np.zeros_like(total_pq)
     for r in range(nprocs):
         h.setup(case.basis, my_rank=r, num_procs=nprocs)
         acc += h.fock_build(case.density)
         q += h.stats()["quartets"]
+        pq += h.stats()["prim_quartets"]
+        if split == 1:  # the reference's split: equal counts, the last rank the remainder
+            assert np.all(np.abs(h.stats()["quartets"] - total_q / nprocs) <= 1)
     h.destroy()
-    assert np.array_equal(q, total_q)
+    assert np.array_equal(q, total_q) and np.array_equal(pq, total_pq)
     assert np.max(np.abs(acc - full)) <= 1e-12
 
 
