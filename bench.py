@@ -38,10 +38,11 @@ sys.path.insert(0, str(ROOT))
 METRIC = "Fock builds/s & ERI quartets/s, FP64 vs mixed @≤1e-6 Eh; serving p50/p99 latency"
 FP64_VALU_PEAK = 78.6e12  # MI355X dense FP64 vector, FLOP/s
 WORKLOADS = {"c60": "tests/golden/cfg_c60.npz", "cn_chain": "tests/golden/cfg_cn_chain.npz",
-             "ethane": "tests/golden/cfg_ethane.npz"}
+             "ethane": "tests/golden/cfg_ethane.npz", "water": "tests/golden/cfg_water.npz"}
 DESCR = {"c60": "C60-like cage, 60 C, s/p/d, 900 basis functions, FP64, Schwarz cutoff 5e-11",
          "cn_chain": "20-atom C/N zig-zag chain, s/p/d/f, 620 basis functions, FP64, Schwarz cutoff 5e-11",
-         "ethane": "ethane-like C2H6, s/p/d, 42 basis functions, FP64, Schwarz cutoff 5e-11"}
+         "ethane": "ethane-like C2H6, s/p/d, 42 basis functions, FP64, Schwarz cutoff 5e-11",
+         "water": "water-like O + 2 H, s/p, 13 basis functions, FP64, Schwarz cutoff 5e-11"}
 
 
 def log(msg: str) -> None:
@@ -368,6 +369,14 @@ def main():
                     "accuracy_vs_reference": {"max_abs_dF": float(np.max(np.abs(ge["fock"] - ce.fock))),
                                               "dE_2e": float(0.5 * np.dot(ce.density, ge["fock"] - ce.fock))},
                     "top_kernels": dict(list(ge["per_class"].items())[:8])}
+        if not a.no_cpu:
+            threads = len(os.sched_getaffinity(0))
+            r = ref_builds(ce, threads, 1)
+            if r:
+                extra[w]["cpu_baseline"] = {"value": round(1.0 / r[0][0], 6), "unit": "Fock builds/s", "cores": threads,
+                                            "kind": "reference", "sample": "the whole workload, one build",
+                                            "max_abs_dF_vs_fixture": float(np.max(np.abs(r[1] - ce.fock)))}
+                log(f"extra workload {w}: reference CPU path {r[0][0]:.2f} s/build on {threads} threads")
     shares = rank_shares(case, a.rank_shares, a.steps) if a.rank_shares > 1 else None
     if shares:
         for k in ("balanced", "equal_counts"):

@@ -44,6 +44,8 @@ def build(verbose: bool = True) -> Path:
     jobs.append([HIPCC, *COMMON, "-DLIBERI_CLASS_GROUP=-1", "-c", str(CSRC / "eri_kernels.hip"), "-o",
                  str(OBJ / "eri_common.o")])
     jobs.append([HIPCC, *COMMON, "-c", str(CSRC / "noalloc.hip"), "-o", str(OBJ / "noalloc.o")])
+    jobs.append([HIPCC, *COMMON, "-DLIBERI_CLASS_GROUP=-3", "-c", str(CSRC / "eri_kernels.hip"), "-o",
+                 str(OBJ / "eri_fused.o")])
     for g in range(KERNEL_GROUPS):
         jobs.append([HIPCC, *COMMON, f"-DLIBERI_CLASS_GROUP={g}", f"-DLIBERI_CLASS_GROUPS={KERNEL_GROUPS}", "-c",
                      str(CSRC / "eri_kernels.hip"), "-o", str(OBJ / f"eri_g{g}.o")])

@@ -112,6 +112,12 @@ struct liberi_ctx {
   DevBuf col_bf[4];     // compact columns per angular momentum -> basis function
   int32_t ncol[4] = {};
   DevBuf aux;           // per build: max |D|, FP32 quartet count
+  // fused group launch: work items, every class's ClassDev, the queue head
+  DevBuf fused_items, fused_cds, fused_head;
+  int64_t fused_n = 0;
+  int fused_nwg = 0;
+  bool fused = false;      // LIBERI_OPT_FUSED
+  bool last_fused = false; // the last build ran the fused launch
   DevBuf tune_F;        // autotuner's scratch Fock matrix
   DevBuf dens_t, dens_b, fock_b;  // UHF: Da + Db, and the beta density / Fock of the host entry point
   hipEvent_t tev[2] = {};
@@ -218,6 +224,10 @@ static void release_device(liberi_ctx *h) {
   dev_free(h->fock);
   dev_free(h->scratch);
   dev_free(h->aux);
+  dev_free(h->fused_items);
+  dev_free(h->fused_cds);
+  dev_free(h->fused_head);
+  h->fused_n = 0;
   dev_free(h->tune_F);
   dev_free(h->dens_t);
   dev_free(h->dens_b);
@@ -320,6 +330,7 @@ int liberi_c_set_option(liberi_ctx *h, int32_t opt, double v) {
       if (v != 0.0 && v != 1.0) return fail(LIBERI_EARG, "liberi_c_set_option: RANK_SPLIT must be 0 or 1");
       h->rank_split = (int)v;
       break;
+    case LIBERI_OPT_FUSED: h->fused = v != 0.0; break;
     case LIBERI_OPT_AUTOTUNE:
       h->autotune = v != 0.0;
       // decide (and time) again at the next build, without earlier handles' choices
@@ -346,6 +357,7 @@ int liberi_c_get_option(liberi_ctx *h, int32_t opt, double *v) {
     case LIBERI_OPT_GRAPH: *v = h->use_graph; break;
     case LIBERI_OPT_MAX_DEVICE_BYTES: *v = h->max_device_bytes; break;
     case LIBERI_OPT_RANK_SPLIT: *v = h->rank_split; break;
+    case LIBERI_OPT_FUSED: *v = h->fused; break;
     default: return fail(LIBERI_EARG, "liberi_c_get_option: unknown option " + std::to_string(opt));
   }
   return LIBERI_OK;
@@ -358,7 +370,10 @@ int liberi_c_get_stats(liberi_ctx *h, liberi_stats *out) {
   for (int c = 0; c < kNumClasses; ++c) {
     const ClassState &cs_ = h->cls[c];
     const bool row = cs_.row_ok && (h->row_tiles == 2 || (h->row_tiles == 1 && cs_.variant == 1));
-    out->class_variant[c] = !cs_.active || cs_.variant < 0 ? -1 : row ? 1 + cs_.dev.transposed : 0;
+    out->class_variant[c] = !cs_.active || cs_.variant < 0       ? -1
+                            : h->last_fused && class_fused_nq(c) > 0 ? 4
+                            : row                                 ? 1 + cs_.dev.transposed
+                                                                  : 0;
     out->class_fp32[c] = cs_.active && h->fp32_tau > 0.0 && cs_.fp32 == 1;
   }
   return LIBERI_OK;
@@ -459,6 +474,44 @@ static void append_pair(std::vector<PairRec> &pairs, std::vector<PrimRec> &prims
       prims.push_back(pr);
     }
   pairs.push_back(r);
+}
+
+// Work items of the fused group launch: every active class that has a body
+// in it, in runs of kFusedRounds rounds of its quartets per item; the classes
+// in order of their primitive quartets (largest first), the items of a class
+// dealt out over its whole quartet range (item k takes run k x stride mod n,
+// stride coprime with n) so that the workgroups running at the same time
+// meet few common Fock rows.
+constexpr int64_t kFusedRounds = 32;
+
+static int setup_fused(liberi_ctx *h) {
+  std::vector<int> cls;
+  for (int c = 0; c < kNumClasses; ++c)
+    if (h->cls[c].active && class_fused_nq(c) > 0 && h->cls[c].dev.q_end > h->cls[c].dev.q_begin) cls.push_back(c);
+  h->fused_n = 0;
+  if (cls.empty()) return LIBERI_OK;
+  std::stable_sort(cls.begin(), cls.end(), [&](int a, int b) { return h->cls[a].nprimq > h->cls[b].nprimq; });
+  std::vector<FusedItem> items;
+  for (int c : cls) {
+    const ClassDev &d = h->cls[c].dev;
+    const int64_t nq = d.q_end - d.q_begin, run = kFusedRounds * class_fused_nq(c);
+    const int64_t n = (nq + run - 1) / run, stride = chunk_stride(n);
+    for (int64_t k = 0; k < n; ++k) {
+      const int64_t j = k * stride % n;
+      items.push_back(FusedItem{c, 0, j * run, std::min(nq, (j + 1) * run)});
+    }
+  }
+  std::vector<ClassDev> cds(kNumClasses);
+  for (int c = 0; c < kNumClasses; ++c) cds[c] = h->cls[c].dev;
+  int rc;
+  if ((rc = dev_upload(h, h->fused_items, items)) || (rc = dev_upload(h, h->fused_cds, cds)) ||
+      (rc = dev_alloc(h, h->fused_head, sizeof(double))))
+    return rc;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess || ncu <= 0) ncu = 256;
+  h->fused_nwg = 12 * ncu;  // three one-wave workgroups per SIMD
+  h->fused_n = (int64_t)items.size();
+  return LIBERI_OK;
 }
 
 // Setup proper, on a handle whose device data is released; the caller
@@ -798,6 +851,7 @@ static int setup_basis(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_b
   h->scratch_per_wg = max_scratch;
   if (max_scratch > 0)
     if ((rc = dev_alloc(h, h->scratch, (size_t)max_scratch * kCoopWorkgroups * sizeof(double)))) return rc;
+  if ((rc = setup_fused(h))) return rc;
   HIP_TRY(hipStreamSynchronize(h->stream));
   h->initialized = true;
   return LIBERI_OK;
@@ -989,9 +1043,16 @@ static int enqueue_build(liberi_ctx *h, const BuildArgs &ba, const double *dD, d
     return fail(LIBERI_EDEVICE, std::string("zero F: ") + hipGetErrorString((hipError_t)e));
   if (h->fp32_tau > 0.0)
     if (int rc = enqueue_dmax(h, dD, st)) return rc;
+  // the fused group launch, where on: its classes leave the per-class launches
+  const bool fused = h->fused && h->fused_n > 0 && !h->class_timing && h->only_class < 0;
+  h->last_fused = fused;
   std::vector<int> order;
+  double fused_cost = 0.0;
   for (int c = 0; c < kNumClasses; ++c)
-    if (h->cls[c].active && (h->only_class < 0 || c == h->only_class)) order.push_back(c);
+    if (h->cls[c].active && (h->only_class < 0 || c == h->only_class)) {
+      if (fused && class_fused_nq(c) > 0) continue;
+      order.push_back(c);
+    }
   std::vector<double> load(ns, 0.0);
   // cost: the class's measured time (first build), else a proxy scaled to
   // milliseconds: primitive quartets x Rys roots x sqrt(cartesian components)
@@ -1006,6 +1067,24 @@ static int enqueue_build(liberi_ctx *h, const BuildArgs &ba, const double *dD, d
     std::sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
     HIP_TRY(hipEventRecord(h->fork_ev, st));
     for (int k = 0; k < ns; ++k) HIP_TRY(hipStreamWaitEvent(h->side[k], h->fork_ev, 0));
+  }
+  if (fused) {
+    // one persistent launch for the fused classes, first onto the least-loaded
+    // stream (the cooperative classes keep side stream 0 to themselves)
+    for (int c = 0; c < kNumClasses; ++c)
+      if (h->cls[c].active && class_fused_nq(c) > 0) fused_cost += cost(c);
+    int k = 0;
+    if (ns > 1) {
+      k = ns > 1 ? 1 : 0;
+      load[k] += fused_cost;
+    }
+    hipStream_t fs = ns > 1 ? h->side[k] : st;
+    int e = launch_fill_zero((double *)h->fused_head.p, 1, fs);
+    if (e == hipSuccess)
+      e = launch_group_fused((const ClassDev *)h->fused_cds.p, ba, (const FusedItem *)h->fused_items.p, h->fused_n,
+                             (unsigned long long *)h->fused_head.p, h->fused_nwg, fs);
+    if (e != hipSuccess)
+      return fail(LIBERI_EDEVICE, std::string("fused group launch: ") + hipGetErrorString((hipError_t)e));
   }
   for (int c : order) {
     hipStream_t cs = st;

@@ -1644,16 +1644,14 @@ __device__ __forceinline__ int gpos_of(int L, int k) {
 // bytes of spill per lane ((dp|dp), (dd|pp), (dd|dp)); measured faster than
 // 2 waves without spill.
 template <int LA, int LB, int LC, int LD>
-__attribute__((amdgpu_waves_per_eu(LA <= 2 && LB <= 2 && LC <= 2 && LD <= 2 && LA + LB + LC + LD <= 7 ? 3 : 1, 8)))
-__global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba) {
+__device__ __forceinline__ void grp_run(const ClassDev &cd, const BuildArgs &ba, int64_t c0, int64_t c1,
+                                        double *lds, int *s_npq, int *s_nk, int *s_boff, int *s_koff) {
   using S = Grp<LA, LB, LC, LD>;
   constexpr int G = S::G, NQ = S::NQ, NR = S::NR, NI = S::NI, NK = S::NK, NE = S::NE, NF = S::NF, NKC = S::NKC;
   constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND, NABCD = S::NABCD;
   constexpr int GSZ = S::GSZ;  // stride of the 2-D tables (padded against bank conflicts)
   constexpr int RT = S::RT, REC = S::REC, KT = S::KT;
   static_assert(RT >= KT && KT * S::GS <= S::UNI, "no room for the tables and rule records");
-  __shared__ double lds[NQ * S::SLOT];
-  __shared__ int s_npq[NQ], s_nk[NQ], s_boff[NQ], s_koff[NQ];
 
   // lanes past the NQ groups (G need not divide 64) idle: their gl lies past
   // every loop bound and their slots are never valid
@@ -2079,17 +2077,10 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
     wave_sync();  // slots are reused by the next round
   };
 
-  // a contiguous chunk of the class's quartets per workgroup (a multiple of
-  // NQ): consecutive rounds stay in the same or the next segment, so each
-  // lane searches the segments once and then steps forward, and they share
-  // the bra pair (its records and density blocks stay in cache)
-  // Workgroups resident at the same time take chunks spread over the whole
-  // quartet range (chunk = workgroup x chunk_stride(grid) mod grid, a
-  // permutation), so their atomics meet fewer common Fock rows.
-  const int64_t nq = cd.q_end - cd.q_begin;
-  const int64_t per = ((nq + gridDim.x - 1) / gridDim.x + NQ - 1) / NQ * NQ;
-  const int64_t chunk = (int64_t)blockIdx.x * cd.chunk_stride % gridDim.x;
-  const int64_t c0 = chunk * per, c1 = c0 + per < nq ? c0 + per : nq;
+  // the quartets [c0, c1) of the class's share, NQ per round: consecutive
+  // rounds stay in the same or the next segment, so each lane searches the
+  // segments once and then steps forward, and they share the bra pair (its
+  // records and density blocks stay in cache)
   int seg = -1;
   for (int64_t base = c0; base < c1; base += NQ) {
     const bool valid = !idle && base + slot < c1;
@@ -2105,8 +2096,26 @@ __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba
   flush_jab();
 }
 
+template <int LA, int LB, int LC, int LD>
+__attribute__((amdgpu_waves_per_eu(LA <= 2 && LB <= 2 && LC <= 2 && LD <= 2 && LA + LB + LC + LD <= 7 ? 3 : 1, 8)))
+__global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba) {
+  using S = Grp<LA, LB, LC, LD>;
+  constexpr int NQ = S::NQ;
+  __shared__ double lds[NQ * S::SLOT];
+  __shared__ int s_npq[NQ], s_nk[NQ], s_boff[NQ], s_koff[NQ];
+  // a contiguous chunk of the class's quartets per workgroup (a multiple of
+  // NQ).  Workgroups resident at the same time take chunks spread over the
+  // whole quartet range (chunk = workgroup x chunk_stride(grid) mod grid, a
+  // permutation), so their atomics meet fewer common Fock rows.
+  const int64_t nq = cd.q_end - cd.q_begin;
+  const int64_t per = ((nq + gridDim.x - 1) / gridDim.x + NQ - 1) / NQ * NQ;
+  const int64_t chunk = (int64_t)blockIdx.x * cd.chunk_stride % gridDim.x;
+  const int64_t c0 = chunk * per, c1 = c0 + per < nq ? c0 + per : nq;
+  grp_run<LA, LB, LC, LD>(cd, ba, c0, c1, lds, s_npq, s_nk, s_boff, s_koff);
+}
+
 // ----------------------------------------------------- segment lengths
-#if !defined(LIBERI_CLASS_GROUP) || LIBERI_CLASS_GROUP < 0
+#if !defined(LIBERI_CLASS_GROUP) || LIBERI_CLASS_GROUP == -1 || LIBERI_CLASS_GROUP == -2
 __global__ void segment_lengths_kernel(const double *bra_x, const int32_t *bra_group, const int32_t *bra_pos,
                                        int32_t nbra, const double *ket_x, const int32_t *gbeg, const int32_t *gend,
                                        int32_t ngroup, int32_t diag, int32_t cmp, double cut, int64_t *seg_len) {
@@ -2427,7 +2436,9 @@ static int64_t scratch_one() {
 
 // The 55 instantiations are spread over LIBERI_CLASS_GROUPS translation units
 // (class id modulo the group count) so they compile in parallel; the unit
-// built with LIBERI_CLASS_GROUP=-1 holds the dispatcher.
+// built with LIBERI_CLASS_GROUP=-1 holds the dispatcher and the common
+// kernels, the one built with -3 the fused group launch; without the macro
+// one unit holds everything.
 #define LIBERI_CLASS_LIST_G0(X) X(0, 0, 0, 0, 0) X(8, 1, 0, 2, 0) X(16, 2, 1, 2, 0) X(24, 0, 0, 3, 2) X(32, 3, 1, 1, 0) X(40, 3, 3, 3, 1) X(48, 3, 3, 0, 0)
 #define LIBERI_CLASS_LIST_G1(X) X(1, 0, 0, 1, 0) X(9, 1, 0, 2, 1) X(17, 2, 1, 1, 1) X(25, 1, 0, 3, 0) X(33, 3, 1, 1, 1) X(41, 3, 2, 1, 1) X(49, 3, 3, 1, 0)
 #define LIBERI_CLASS_LIST_G2(X) X(2, 0, 0, 1, 1) X(10, 0, 0, 2, 2) X(18, 2, 2, 2, 0) X(26, 3, 3, 3, 0) X(34, 3, 1, 2, 0) X(42, 3, 2, 2, 0) X(50, 3, 3, 1, 1)
@@ -2478,7 +2489,92 @@ LIBERI_CLASS_LIST_G7(DEF_CLASS)
 #endif
 #undef DEF_CLASS
 
-#if LIBERI_CLASS_GROUP < 0
+// ------------------------------------------------- fused group launch
+// The s, p, d classes of the group path that run three waves per SIMD (all
+// but (dd|dd)) in one persistent launch fed from a device work queue: items
+// (class, range of quartets) in an array, taken by the workgroups one at a
+// time from an atomic counter, so the classes of a build share the grid and
+// no class waits for another's tail.  Each item runs the class's group
+// kernel body on one wave; the LDS is the largest slot set among the classes.
+#define LIBERI_FUSED_LIST(X) \
+  X(11, 1, 0, 2, 2) X(14, 2, 2, 1, 1) X(15, 2, 1, 2, 1) X(16, 2, 1, 2, 0) X(17, 2, 1, 1, 1) X(18, 2, 2, 2, 0) X(19, 2, 2, 2, 1)
+
+#if LIBERI_CLASS_GROUP == -3 || LIBERI_CLASS_GROUP == -2
+constexpr int fused_lds_doubles() {
+  int m = 0;
+#define X(id, a, b, cc, d) m = Grp<a, b, cc, d>::NQ * Grp<a, b, cc, d>::SLOT > m ? Grp<a, b, cc, d>::NQ * Grp<a, b, cc, d>::SLOT : m;
+  LIBERI_FUSED_LIST(X)
+#undef X
+  return m;
+}
+constexpr int fused_nq() {
+  int m = 0;
+#define X(id, a, b, cc, d) m = Grp<a, b, cc, d>::NQ > m ? Grp<a, b, cc, d>::NQ : m;
+  LIBERI_FUSED_LIST(X)
+#undef X
+  return m;
+}
+#define X(id, a, b, cc, d)                                                                                    \
+  static_assert(!use_small<a, b, cc, d>() && Grp<a, b, cc, d>::fits && a + b + cc + d <= 7 && a <= 2 && b <= 2 && \
+                    cc <= 2 && d <= 2,                                                                         \
+                "fused classes are group classes at three waves per SIMD");
+LIBERI_FUSED_LIST(X)
+#undef X
+
+// one item's class body as a call of its own, so that each class keeps its
+// own register allocation instead of one allocation over all of them
+template <int LA, int LB, int LC, int LD>
+__device__ __noinline__ void grp_run_item(const ClassDev &cd, const BuildArgs &ba, int64_t c0, int64_t c1,
+                                          double *lds, int *s_npq, int *s_nk, int *s_boff, int *s_koff) {
+  grp_run<LA, LB, LC, LD>(cd, ba, c0, c1, lds, s_npq, s_nk, s_boff, s_koff);
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void eri_group_fused_kernel(
+    const ClassDev *cds, BuildArgs ba, const FusedItem *items, int64_t nitems, unsigned long long *head) {
+  __shared__ double lds[fused_lds_doubles()];
+  __shared__ int s_npq[fused_nq()], s_nk[fused_nq()], s_boff[fused_nq()], s_koff[fused_nq()];
+  for (;;) {
+    unsigned long long it = 0;
+    if (threadIdx.x == 0) it = atomicAdd(head, 1ull);
+    it = __shfl(it, 0, 64);
+    if (it >= (unsigned long long)nitems) break;
+    const FusedItem w = items[it];
+    switch (w.cls) {
+#define X(id, a, b, cc, d)                                                                 \
+  case id:                                                                                 \
+    grp_run_item<a, b, cc, d>(cds[id], ba, w.c0, w.c1, lds, s_npq, s_nk, s_boff, s_koff); \
+    break;
+      LIBERI_FUSED_LIST(X)
+#undef X
+      default:
+        break;
+    }
+    wave_sync();  // the next item reuses the slots
+  }
+}
+
+int launch_group_fused(const ClassDev *cds, const BuildArgs &ba, const FusedItem *items, int64_t nitems,
+                       unsigned long long *head, int nwg, void *stream) {
+  if (nitems <= 0) return hipSuccess;
+  hipLaunchKernelGGL(eri_group_fused_kernel, dim3((unsigned)nwg), dim3(64), 0, (hipStream_t)stream, cds, ba, items,
+                     nitems, head);
+  return hipGetLastError();
+}
+
+int class_fused_nq(int c) {
+  switch (c) {
+#define X(id, a, b, cc, d) \
+  case id:                 \
+    return Grp<a, b, cc, d>::NQ;
+    LIBERI_FUSED_LIST(X)
+#undef X
+    default:
+      return 0;
+  }
+}
+#endif
+
+#if LIBERI_CLASS_GROUP == -1 || LIBERI_CLASS_GROUP == -2
 int launch_class(int c, const ClassDev &cd, const BuildArgs &ba, void *stream, int nwg) {
   hipStream_t st = (hipStream_t)stream;
   if (!(cd.xswitch <= class_rys_xmax(c))) return hipErrorInvalidValue;

@@ -137,6 +137,19 @@ int64_t class_row_lds(int c, int transposed, const int32_t *ncol, double xswitch
 // kernel shape of class c: 0 lane per quartet, 1 workgroup group per
 // quartet, 2 cooperative workgroup per quartet
 int class_kernel_kind(int c);
+
+// Fused group launch: one work item = quartets [c0, c1) of class cls (relative
+// to the class's share, c0 a multiple of its quartets per round)
+struct FusedItem {
+  int32_t cls, pad;
+  int64_t c0, c1;
+};
+// quartets per round of class c in the fused group launch (0: not fused)
+int class_fused_nq(int c);
+// the items[0 .. nitems) of the build, taken through the counter *head
+// (zero before the launch); cds: ClassDev of every class by class index
+int launch_group_fused(const ClassDev *cds, const BuildArgs &ba, const FusedItem *items, int64_t nitems,
+                       unsigned long long *head, int nwg, void *stream);
 // longest row tile, quartets
 constexpr int64_t kTileQuartets = 4096;
 // LDS the row-tiled kernel may take per workgroup

@@ -41,6 +41,7 @@ OPT_AUTOTUNE = 9
 OPT_GRAPH = 10
 OPT_MAX_DEVICE_BYTES = 11
 OPT_RANK_SPLIT = 12
+OPT_FUSED = 13
 
 _STATUS = {1: "EARG", 2: "ESTATE", 3: "ESIZE", 4: "EDEVICE", 5: "ENOMEM"}
 

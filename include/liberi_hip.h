@@ -158,6 +158,10 @@ const char *liberi_c_last_error(void);
                                         a per-quartet term); 1 equal quartet counts, the reference's split
                                         (e.g. src/rhf/rot_axis/int0001.F90:100-104).  Either way the ranks'
                                         partial Fock matrices sum to the full one.  Read at setup. */
+#define LIBERI_OPT_FUSED 13          /* 1 (default): the s, p, d classes of the group path run in one
+                                        persistent launch fed from a device work queue (no class waits
+                                        for another's tail); 0: one launch per class.  Per-class timing
+                                        and ONLY_CLASS builds launch per class.  Read at each build. */
 #define LIBERI_OPT_MAX_DEVICE_BYTES 11 /* HBM budget of the handle in bytes (0, the default: none).  An
                                           allocation that would take the handle's device data past it
                                           fails with LIBERI_ENOMEM; a setup that fails that way (or any
@@ -179,9 +183,10 @@ typedef struct {
                                            LIBERI_OPT_CLASS_TIMING is on (else 0) */
   int64_t prim_pairs;                   /* primitive pairs kept on the device */
   int64_t prim_pairs_dropped;           /* primitive pairs removed by LIBERI_OPT_PRIM_CUTOFF */
-  int32_t class_variant[LIBERI_NCLASS]; /* digestion used per class: 0 direct atomics, 1 row tiles
-                                           over bra pairs, 2 row tiles over ket pairs; -1 class
-                                           inactive or not yet built */
+  int32_t class_variant[LIBERI_NCLASS]; /* kernel form used per class in the last build: 0 its own
+                                           launch with direct atomics, 1 row tiles over bra pairs,
+                                           2 row tiles over ket pairs, 4 inside the fused group launch
+                                           (LIBERI_OPT_FUSED); -1 class inactive or not yet built */
   int32_t class_fp32[LIBERI_NCLASS];    /* 1: the mixed form (LIBERI_OPT_FP32_TAU > 0) evaluates this
                                            class's qualifying quartets in FP32, chosen per class from
                                            measured time at the first build under that tau; 0: FP64 */

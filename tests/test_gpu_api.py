@@ -329,3 +329,29 @@ def _pinned(a):
 
 
 _pinned.keep = []
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["cfg_ethane.npz", "cfg_cn_chain.npz", "cfg_c60.npz"])
+def test_fused_group_launch_matches_per_class_launches(name):
+    """LIBERI_OPT_FUSED: the s, p, d group classes in one persistent launch fed
+    from a device work queue give the per-class launches' Fock matrix (to
+    summation order), graph-captured and replayed as well as eager."""
+    case = Case.load_npz(GOLD / name)
+    out = {}
+    for fused in (0, 1):
+        for graph in (0, 1):
+            h = liberi.Handle()
+            h.set_option(liberi.OPT_FUSED, fused)
+            h.set_option(liberi.OPT_GRAPH, graph)
+            h.setup(case.basis)
+            f = [h.fock_build(case.density) for _ in range(3)]
+            v = h.stats()["class_variant"]
+            h.destroy()
+            assert np.any(v == 4) == bool(fused)  # every case has fused classes
+            out[(fused, graph)] = f
+    ref = out[(0, 0)][0]
+    scale = max(1.0, float(np.max(np.abs(ref))))
+    for k, fs in out.items():
+        for f in fs:
+            assert np.max(np.abs(f - ref)) <= 1e-11 * scale, k
