@@ -169,7 +169,7 @@ def gpu_leg(case, steps: int, warmup: int, taus=(), graph: bool = True, host_ent
     total_flops = float(flops.sum())
     per_class = {census.NAMES[c]: {"ms": round(float(class_ms[c]), 4), "quartets": int(q[c]),
                                    "tflops": round(flops[c] / max(class_ms[c], 1e-9) * 1e-9, 3),
-                                   "form": int(st["class_variant"][c])}
+                                   "form": int(st["class_variant"][c]), "tile": int(st["class_tile"][c])}
                  for c in np.argsort(-class_ms) if class_ms[c] > 0}
     return dict(ms_per_step=ms_per_step, fock=fock, stats=st, dom=dom, achieved=achieved,
                 dom_ms=float(class_ms[dom]), dom_flops=float(flops[dom]), total_flops=total_flops,
@@ -219,7 +219,7 @@ def cpu_leg(case, reps: int):
         if rr:
             scal[str(t)] = rr[0][0]
     out["scaling_frag_c12_s"] = scal
-    log(f"cpu baseline: {out['seconds']:.2f} s/build (median of {secs}); c12 piece by threads {scal}")
+    log(f"cpu baseline: {out['seconds']:.4g} s/build (median of {secs}); c12 piece by threads {scal}")
     return out
 
 
@@ -390,7 +390,7 @@ def main():
                     "cores": cpu["threads"], "kind": "reference",
                     "sample": (f"the whole workload: reference CPU path (flang build of gms_libERI, OpenMP host "
                                f"fallback) on {cpu['threads']} threads, median of {len(cpu['all_seconds'])} builds "
-                               f"{[round(x, 2) for x in cpu['all_seconds']]} s; max|F - fixture| "
+                               f"{[float('%.4g' % x) for x in cpu['all_seconds']]} s; max|F - fixture| "
                                f"{cpu['max_abs_dF_vs_fixture']:.1e}"),
                     "quartets_per_s": round(q_total / cpu["seconds"], 1),
                     "thread_scaling_frag_c12_s": cpu["scaling_frag_c12_s"]}

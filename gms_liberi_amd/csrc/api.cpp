@@ -88,9 +88,21 @@ struct ClassState {
   float ms = -1.f;      // device time of the class's launch, measured at the first build
   int64_t nquart = 0, nprimq = 0;
   int32_t nseg = 0;
-  DevBuf ket_list, seg_start, seg_bra, seg_ket0, tile_q, tile_seg;
+  int tile = -1;        // row-tile length chosen for the class, index into kTileLens (-1: not yet)
+  DevBuf ket_list, seg_start, seg_bra, seg_ket0;
+  // the row tiles of this rank's share, for each tile length
+  DevBuf tile_q[kNumTileLens], tile_seg[kNumTileLens];
+  int32_t t_begin[kNumTileLens] = {}, t_end[kNumTileLens] = {};
   ClassDev dev{};
 };
+
+// Point the class's launch arguments at its row tiles of length kTileLens[k].
+static void use_tiles(ClassState &cs_, int k) {
+  cs_.dev.tile_q = (const int64_t *)cs_.tile_q[k].p;
+  cs_.dev.tile_seg = (const int32_t *)cs_.tile_seg[k].p;
+  cs_.dev.t_begin = cs_.t_begin[k];
+  cs_.dev.t_end = cs_.t_end[k];
+}
 
 struct liberi_ctx {
   // options
@@ -117,6 +129,7 @@ struct liberi_ctx {
   int64_t fused_n = 0;
   int fused_nwg = 0;
   bool fused = false;      // LIBERI_OPT_FUSED
+  int tile_len = -1;       // LIBERI_OPT_TILE_LEN as an index into kTileLens; -1 chosen per class
   bool last_fused = false; // the last build ran the fused launch
   DevBuf tune_F;        // autotuner's scratch Fock matrix
   DevBuf dens_t, dens_b, fock_b;  // UHF: Da + Db, and the beta density / Fock of the host entry point
@@ -214,8 +227,7 @@ static void release_device(liberi_ctx *h) {
     dev_free(c.seg_start);
     dev_free(c.seg_bra);
     dev_free(c.seg_ket0);
-    dev_free(c.tile_q);
-    dev_free(c.tile_seg);
+    for (int k = 0; k < kNumTileLens; ++k) dev_free(c.tile_q[k]), dev_free(c.tile_seg[k]);
     c = ClassState{};
   }
   dev_free(h->pairs);
@@ -331,10 +343,20 @@ int liberi_c_set_option(liberi_ctx *h, int32_t opt, double v) {
       h->rank_split = (int)v;
       break;
     case LIBERI_OPT_FUSED: h->fused = v != 0.0; break;
+    case LIBERI_OPT_TILE_LEN: {
+      int k = -1;
+      for (int i = 0; i < kNumTileLens; ++i)
+        if (v == (double)kTileLens[i]) k = i;
+      if (v != 0.0 && k < 0)
+        return fail(LIBERI_EARG, "liberi_c_set_option: TILE_LEN must be 0, 1024, 4096 or 16384");
+      h->tile_len = v == 0.0 ? -1 : k;
+      for (auto &c : h->cls) c.tile = -1;  // chosen again at the next build
+      break;
+    }
     case LIBERI_OPT_AUTOTUNE:
       h->autotune = v != 0.0;
       // decide (and time) again at the next build, without earlier handles' choices
-      for (auto &c : h->cls) c.variant = -1, c.fp32 = -1, c.ms = -1.f;
+      for (auto &c : h->cls) c.variant = -1, c.fp32 = -1, c.tile = -1, c.ms = -1.f;
       forget_tuned_choices();
       break;
     default: return fail(LIBERI_EARG, "liberi_c_set_option: unknown option " + std::to_string(opt));
@@ -358,6 +380,7 @@ int liberi_c_get_option(liberi_ctx *h, int32_t opt, double *v) {
     case LIBERI_OPT_MAX_DEVICE_BYTES: *v = h->max_device_bytes; break;
     case LIBERI_OPT_RANK_SPLIT: *v = h->rank_split; break;
     case LIBERI_OPT_FUSED: *v = h->fused; break;
+    case LIBERI_OPT_TILE_LEN: *v = h->tile_len < 0 ? 0.0 : (double)kTileLens[h->tile_len]; break;
     default: return fail(LIBERI_EARG, "liberi_c_get_option: unknown option " + std::to_string(opt));
   }
   return LIBERI_OK;
@@ -375,6 +398,9 @@ int liberi_c_get_stats(liberi_ctx *h, liberi_stats *out) {
                             : row                                 ? 1 + cs_.dev.transposed
                                                                   : 0;
     out->class_fp32[c] = cs_.active && h->fp32_tau > 0.0 && cs_.fp32 == 1;
+    out->class_tile[c] = cs_.active && row && cs_.tile >= 0 && !(h->last_fused && class_fused_nq(c) > 0)
+                             ? (int32_t)kTileLens[cs_.tile]
+                             : 0;
   }
   return LIBERI_OK;
 }
@@ -747,9 +773,8 @@ static int setup_basis(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_b
     cleanup_tmp();
     if (e != hipSuccess)
       return fail(LIBERI_EDEVICE, std::string("segment generation: ") + hipGetErrorString((hipError_t)e));
-    std::vector<int64_t> sstart, tq;
-    std::vector<int32_t> sbra, sket0, tseg;
-    const int64_t tile_len = kTileQuartets;
+    std::vector<int64_t> sstart, tq[kNumTileLens];
+    std::vector<int32_t> sbra, sket0, tseg[kNumTileLens];
     int64_t total = 0, primq = 0;
     for (size_t i = 0; i < kb.size(); ++i) {
       const int64_t row0 = total;
@@ -763,26 +788,29 @@ static int setup_basis(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_b
         total += L;
         primq += L * (int64_t)pairs[kb[i]].nprim * pairs[kk[gbeg[g]]].nprim;
       }
-      // row tiles: the row's quartets in runs of at most tile_len
-      size_t sg = sfirst;
-      for (int64_t t0 = row0; t0 < total; t0 += tile_len) {
-        while (sg + 1 < sstart.size() && sstart[sg + 1] <= t0) ++sg;
-        tq.push_back(t0);
-        tseg.push_back((int32_t)sg);
+      // row tiles: the row's quartets in runs of at most kTileLens[k]
+      for (int k = 0; k < kNumTileLens; ++k) {
+        size_t sg = sfirst;
+        for (int64_t t0 = row0; t0 < total; t0 += kTileLens[k]) {
+          while (sg + 1 < sstart.size() && sstart[sg + 1] <= t0) ++sg;
+          tq[k].push_back(t0);
+          tseg[k].push_back((int32_t)sg);
+        }
       }
     }
     sstart.push_back(total);
-    tq.push_back(total);
+    for (auto &t : tq) t.push_back(total);
     cs_.nquart = total;
     cs_.nprimq = primq;
     if (total == 0) continue;
-    if (sbra.size() >= (size_t)INT32_MAX || tq.size() >= (size_t)INT32_MAX)
+    if (sbra.size() >= (size_t)INT32_MAX || tq[0].size() >= (size_t)INT32_MAX)
       return fail(LIBERI_ESIZE, std::string("class ") + def.name + ": too many segments");
     cs_.nseg = (int32_t)sbra.size();
     if ((rc = dev_upload(h, cs_.ket_list, kk)) || (rc = dev_upload(h, cs_.seg_start, sstart)) ||
-        (rc = dev_upload(h, cs_.seg_bra, sbra)) || (rc = dev_upload(h, cs_.seg_ket0, sket0)) ||
-        (rc = dev_upload(h, cs_.tile_q, tq)) || (rc = dev_upload(h, cs_.tile_seg, tseg)))
+        (rc = dev_upload(h, cs_.seg_bra, sbra)) || (rc = dev_upload(h, cs_.seg_ket0, sket0)))
       return rc;
+    for (int k = 0; k < kNumTileLens; ++k)
+      if ((rc = dev_upload(h, cs_.tile_q[k], tq[k])) || (rc = dev_upload(h, cs_.tile_seg[k], tseg[k]))) return rc;
     cs_.active = true;
     ClassDev &d = cs_.dev;
     d.bra_list = nullptr;
@@ -833,13 +861,15 @@ static int setup_basis(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_b
       primq = pq;
     }
     // tiles overlapping [q_begin, q_end): tq is strictly increasing, tq.back() = total
-    const int32_t ntile = (int32_t)tq.size() - 1;
-    d.tile_q = (const int64_t *)cs_.tile_q.p;
-    d.tile_seg = (const int32_t *)cs_.tile_seg.p;
-    d.t_begin = (int32_t)(std::upper_bound(tq.begin(), tq.end(), d.q_begin) - tq.begin()) - 1;
-    d.t_end = (int32_t)(std::lower_bound(tq.begin(), tq.end(), d.q_end) - tq.begin());
-    d.t_begin = std::max<int32_t>(0, std::min(d.t_begin, ntile));
-    d.t_end = std::max(d.t_begin, std::min(d.t_end, ntile));
+    for (int k = 0; k < kNumTileLens; ++k) {
+      const std::vector<int64_t> &t = tq[k];
+      const int32_t ntile = (int32_t)t.size() - 1;
+      int32_t b = (int32_t)(std::upper_bound(t.begin(), t.end(), d.q_begin) - t.begin()) - 1;
+      int32_t e = (int32_t)(std::lower_bound(t.begin(), t.end(), d.q_end) - t.begin());
+      cs_.t_begin[k] = b = std::max<int32_t>(0, std::min(b, ntile));
+      cs_.t_end[k] = std::max(b, std::min(e, ntile));
+    }
+    use_tiles(cs_, kDefaultTile);
     d.row_mode = 0;
     d.transposed = transposed;
     d.xswitch = h->exact_large_t ? class_rys_xmax(c) : reference_switch(c);
@@ -981,6 +1011,40 @@ static int choose_variants(liberi_ctx *h, const BuildArgs &ba, hipStream_t st) {
     cs_.variant = best[1] < best[0] ? 1 : 0;
     cs_.ms = std::min(best[0], best[1]);
     set_tuned_choice(c, nq, -1.0, cs_.variant);
+  }
+  // row-tile length of the row-tiled classes: shorter tiles give more
+  // workgroups and more flushes of the exchange rows, longer ones fewer
+  // workgroups (rows longer than a tile are split)
+  for (int c = 0; c < kNumClasses; ++c) {
+    ClassState &cs_ = h->cls[c];
+    if (!cs_.active || cs_.tile >= 0) continue;
+    const int64_t nq = cs_.dev.q_end - cs_.dev.q_begin;
+    cs_.tile = h->tile_len >= 0 ? h->tile_len : kDefaultTile;
+    if (h->tile_len >= 0 || !h->autotune || !class_row_mode(h, c) || nq < kTuneMinQuartets ||
+        (h->only_class >= 0 && c != h->only_class)) {
+      use_tiles(cs_, cs_.tile);
+      continue;
+    }
+    if (const int prev = tuned_choice(c, nq, -2.0); prev >= 0) {
+      cs_.tile = prev;
+      use_tiles(cs_, cs_.tile);
+      continue;
+    }
+    float best[kNumTileLens];
+    for (int k = 0; k < kNumTileLens; ++k) best[k] = 1e30f;
+    for (int rep = 0; rep < 2; ++rep)
+      for (int k = 0; k < kNumTileLens; ++k) {
+        if (cs_.t_end[k] <= cs_.t_begin[k]) continue;
+        use_tiles(cs_, k);
+        float ms = 0.f;
+        if (int rc = time_class(h, ba, st, c, true, false, ms)) return rc;
+        best[k] = std::min(best[k], ms);
+      }
+    for (int k = 0; k < kNumTileLens; ++k)
+      if (best[k] < best[cs_.tile]) cs_.tile = k;
+    use_tiles(cs_, cs_.tile);
+    cs_.ms = best[cs_.tile];
+    set_tuned_choice(c, nq, -2.0, cs_.tile);
   }
   // mixed form: FP32 stays on only in the classes where it is measured
   // faster than FP64 by at least kFp32Gain (without packed FP32 an FP32

@@ -83,7 +83,7 @@ struct ClassDev {
   int32_t diag;
   int64_t q_begin, q_end;    // this rank's share of the class's quartets
   double xswitch;            // Rys argument from which the asymptotic rule is used
-  // Row tiles: runs of at most kTileQuartets quartets of one row.  Tile t
+  // Row tiles: runs of at most the class's tile length (kTileLens) of one row.  Tile t
   // covers [tile_q[t], tile_q[t+1]) and starts in segment tile_seg[t]; this
   // rank's tiles are [t_begin, t_end).  row_mode selects the row-tiled kernel.
   const int64_t *tile_q;     // ntile + 1
@@ -151,7 +151,11 @@ int class_fused_nq(int c);
 int launch_group_fused(const ClassDev *cds, const BuildArgs &ba, const FusedItem *items, int64_t nitems,
                        unsigned long long *head, int nwg, void *stream);
 // longest row tile, quartets
-constexpr int64_t kTileQuartets = 4096;
+// row-tile lengths (quartets of one row per workgroup) the autotuner times
+// per row-tiled class; kDefaultTile where it does not time
+constexpr int kNumTileLens = 3;
+constexpr int64_t kTileLens[kNumTileLens] = {1024, 4096, 16384};
+constexpr int kDefaultTile = 1;
 // LDS the row-tiled kernel may take per workgroup
 constexpr int64_t kRowLdsBytes = 64 * 1024;
 // workgroups the cooperative kernels use

@@ -42,6 +42,7 @@ OPT_GRAPH = 10
 OPT_MAX_DEVICE_BYTES = 11
 OPT_RANK_SPLIT = 12
 OPT_FUSED = 13
+OPT_TILE_LEN = 14
 
 _STATUS = {1: "EARG", 2: "ESTATE", 3: "ESIZE", 4: "EDEVICE", 5: "ENOMEM"}
 
@@ -57,7 +58,7 @@ class Stats(C.Structure):
                 ("total_quartets", C.c_int64), ("fp32_quartets", C.c_int64), ("last_build_ms", C.c_double),
                 ("device_bytes", C.c_int64), ("class_ms", C.c_double * NCLASS), ("prim_pairs", C.c_int64),
                 ("prim_pairs_dropped", C.c_int64), ("class_variant", C.c_int32 * NCLASS),
-                ("class_fp32", C.c_int32 * NCLASS)]
+                ("class_fp32", C.c_int32 * NCLASS), ("class_tile", C.c_int32 * NCLASS)]
 
 
 _lib = None
@@ -231,7 +232,8 @@ class Handle:
                 "class_ms": np.array(s.class_ms[:], np.float64), "prim_pairs": s.prim_pairs,
                 "prim_pairs_dropped": s.prim_pairs_dropped,
                 "class_variant": np.array(s.class_variant[:], dtype=np.int32),
-                "class_fp32": np.array(s.class_fp32[:], dtype=np.int32)}
+                "class_fp32": np.array(s.class_fp32[:], dtype=np.int32),
+                "class_tile": np.array(s.class_tile[:], dtype=np.int32)}
 
 
 # ---------------------------------------------- reference-named functions

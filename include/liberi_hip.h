@@ -158,10 +158,15 @@ const char *liberi_c_last_error(void);
                                         a per-quartet term); 1 equal quartet counts, the reference's split
                                         (e.g. src/rhf/rot_axis/int0001.F90:100-104).  Either way the ranks'
                                         partial Fock matrices sum to the full one.  Read at setup. */
-#define LIBERI_OPT_FUSED 13          /* 1 (default): the s, p, d classes of the group path run in one
-                                        persistent launch fed from a device work queue (no class waits
-                                        for another's tail); 0: one launch per class.  Per-class timing
-                                        and ONLY_CLASS builds launch per class.  Read at each build. */
+#define LIBERI_OPT_FUSED 13          /* 0 (default): one launch per class; 1: the seven s, p, d classes
+                                        of the group path with the smallest LDS slots run in one
+                                        persistent launch fed from a device work queue (slower on the
+                                        workloads measured).  Per-class timing and ONLY_CLASS builds
+                                        launch per class.  Read at each build. */
+#define LIBERI_OPT_TILE_LEN 14       /* row-tile length of the row-tiled classes, in quartets of one row
+                                        per workgroup: 0 (default) timed per class at the first build
+                                        (LIBERI_OPT_AUTOTUNE; 4096 where untimed), or 1024, 4096, 16384
+                                        for every class.  Read at each build. */
 #define LIBERI_OPT_MAX_DEVICE_BYTES 11 /* HBM budget of the handle in bytes (0, the default: none).  An
                                           allocation that would take the handle's device data past it
                                           fails with LIBERI_ENOMEM; a setup that fails that way (or any
@@ -190,6 +195,10 @@ typedef struct {
   int32_t class_fp32[LIBERI_NCLASS];    /* 1: the mixed form (LIBERI_OPT_FP32_TAU > 0) evaluates this
                                            class's qualifying quartets in FP32, chosen per class from
                                            measured time at the first build under that tau; 0: FP64 */
+  int32_t class_tile[LIBERI_NCLASS];    /* row-tile length (quartets of one row per workgroup: 1024,
+                                           4096 or 16384) of a row-tiled class, timed per class at the
+                                           first build after setup (LIBERI_OPT_AUTOTUNE) or 4096;
+                                           0 where the class is not row-tiled */
 } liberi_stats;
 int liberi_c_get_stats(liberi_ctx *ctx, liberi_stats *out);
 

@@ -355,3 +355,36 @@ def test_fused_group_launch_matches_per_class_launches(name):
     for k, fs in out.items():
         for f in fs:
             assert np.max(np.abs(f - ref)) <= 1e-11 * scale, k
+
+
+@pytest.mark.gpu
+def test_row_tile_lengths_give_the_same_fock():
+    """Every row-tile length (LIBERI_OPT_TILE_LEN) and the per-class timed
+    choice give the reference's Fock matrix, equal to summation order; the
+    statistics report the length each row-tiled class ran with."""
+    case = Case.load_npz(GOLD / "cfg_c60.npz")
+    h = liberi.Handle()
+    h.setup(case.basis)
+    out = {}
+    for length in (0, 1024, 4096, 16384):
+        h.set_option(liberi.OPT_TILE_LEN, length)
+        assert h.get_option(liberi.OPT_TILE_LEN) == length
+        out[length] = h.fock_build(case.density)
+        st = h.stats()
+        row = (st["class_variant"] == 1) | (st["class_variant"] == 2)
+        assert row.sum() >= 10
+        assert np.all(st["class_tile"][~row] == 0)
+        if length:
+            assert np.all(st["class_tile"][row] == length)
+        else:
+            assert set(st["class_tile"][row].tolist()) <= {1024, 4096, 16384}
+            print("tile lengths chosen:", {liberi.class_names()[c]: int(st["class_tile"][c])
+                                           for c in np.flatnonzero(row)})
+    with pytest.raises(liberi.LiberiError):
+        h.set_option(liberi.OPT_TILE_LEN, 2048)
+    h.destroy()
+    scale = max(1.0, float(np.max(np.abs(out[0]))))
+    for length, f in out.items():
+        assert np.max(np.abs(f - out[0])) <= 1e-13 * scale, length
+    # F against the reference class by class: test_gpu_parity
+    assert abs(0.5 * float(np.dot(case.density, out[0] - case.fock))) <= 1e-9

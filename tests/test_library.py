@@ -25,6 +25,19 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_python_mirror_matches_the_header():
+    """Option numbers and the liberi_stats layout of the Python mirror are the
+    header's (a field missing in the mirror would shift every later one)."""
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    opts = dict(re.findall(r"#define LIBERI_(OPT_\w+)\s+(\d+)", text))
+    assert len(opts) >= 14
+    for name, val in opts.items():
+        assert getattr(liberi, name) == int(val), name
+    body = re.search(r"typedef struct\s*\{(.*?)\}\s*liberi_stats;", text, flags=re.S).group(1)
+    fields = re.findall(r"\b(\w+)(?:\[\w+\])?;", body)
+    assert [f[0] for f in liberi.Stats._fields_] == fields
+
+
 def test_class_tables_agree(oracle):
     assert liberi.class_names() == census.NAMES == oracle.class_names
     assert len(census.CLASS_L) == 55
