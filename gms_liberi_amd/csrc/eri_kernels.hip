@@ -914,6 +914,32 @@ __device__ __forceinline__ int range_index(int L0, int lx, int ly, int lz) {
   return nrange(L0, lx + ly + lz - 1) + cidx(lx, ly, lz);
 }
 
+// component k (internal order) of a shell of angular momentum L -> GAMESS
+// position and cartesian scale factor, without loops
+__device__ __forceinline__ int gpos_rt(int L, int k) {
+  if (L <= 1) return k;
+  if (L == 2) return (int)((0x251430u >> (4 * k)) & 15u);
+  return (int)((0x2861795430ull >> (4 * k)) & 15ull);
+}
+__device__ __forceinline__ double scale_rt(int L, int k) {
+  if (L == 2) return (k == 1 || k == 2 || k == 4) ? 1.7320508075688772935 : 1.0;
+  if (L == 3) return (k == 0 || k == 6 || k == 9) ? 1.0 : (k == 4 ? 3.8729833462074168852 : 2.2360679774997896964);
+  return 1.0;
+}
+
+// Workgroup-per-quartet path for the classes whose group slot exceeds the
+// LDS (the f classes).  One workgroup of NT lanes evaluates one quartet at a
+// time: 3 NR lanes make the 2-D tables of a primitive quartet, every lane
+// accumulates its components of (e0|f0); the HRR levels, the cartesian
+// scales and the digestion then run over all lanes.  The HRR buffers live in
+// LDS wherever the class's two buffers, tables, density blocks and step
+// tables fit kCoopLds (gfx950: 160 KB per workgroup), else in a per-workgroup
+// HBM scratch.  Each HRR level reads a per-row step table (source rows and
+// direction, made once per workgroup), so an item costs two reads, one FMA
+// and one write; the digestion reads the six density blocks from LDS with
+// compile-time strides.
+constexpr int kCoopLds = 160 * 1024;
+
 template <int LA, int LB, int LC, int LD>
 struct Coop {
   static constexpr int LAB = LA + LB, LCD = LC + LD;
@@ -922,21 +948,25 @@ struct Coop {
   static constexpr int NE = nrange(LA, LAB), NF = nrange(LC, LCD);
   static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   static constexpr int NABCD = NA * NB * NC * ND;
-  static constexpr int NT = NE * NF > 2048 ? 256 : (NE * NF > 512 ? 128 : 64);
-  static constexpr int NKC = (NE * NF + NT - 1) / NT;  // accumulators per lane
-  static constexpr int64_t buf_doubles() {
-    int64_t m = (int64_t)NE * NF;
-    for (int j = 1; j <= LB; ++j) {
-      int64_t s = (int64_t)nrange(LA, LAB - j) * ncart(j) * NF;
-      m = s > m ? s : m;
-    }
-    for (int j = 1; j <= LD; ++j) {
-      int64_t s = (int64_t)NA * NB * nrange(LC, LCD - j) * ncart(j);
-      m = s > m ? s : m;
-    }
-    return m;
+  static constexpr int NAB = NA * NB, NCD = NC * ND, NAC = NA * NC, NBD = NB * ND, NAD = NA * ND, NBC = NB * NC;
+  static constexpr int DB = NAB + NCD + NAC + NBD + NAD + NBC;
+  static constexpr int GT = 3 * NR * NI * NK;  // 2-D tables of one primitive quartet
+  static constexpr int64_t buf_doubles() { return hrr_max_buffer(LA, LB, LC, LD); }
+  // step-table entries: bra levels 1..LB (rows a, b), then ket levels 1..LD (c, d)
+  static constexpr int bra_rows(int j) { return nrange(LA, LAB - j) * ncart(j); }
+  static constexpr int ket_rows(int j) { return nrange(LC, LCD - j) * ncart(j); }
+  static constexpr int tab_off(int lev) {  // lev 0..LB-1 bra, LB..LB+LD-1 ket
+    int o = 0;
+    for (int i = 0; i < lev; ++i) o += i < LB ? bra_rows(i + 1) : ket_rows(i - LB + 1);
+    return o;
   }
-  static constexpr int64_t scratch() { return 2 * buf_doubles(); }
+  static constexpr int NTAB = tab_off(LB + LD);
+  static constexpr int fixed_doubles() { return GT + DB + (NTAB + 1) / 2; }
+  static constexpr bool IN_LDS = (int64_t)8 * (fixed_doubles() + 2 * buf_doubles()) <= kCoopLds;
+  static constexpr int LDS_DOUBLES = fixed_doubles() + (IN_LDS ? 2 * (int)buf_doubles() : 0);
+  static constexpr int NT = IN_LDS || NE * NF > 2048 ? 512 : (NE * NF > 512 ? 128 : 64);
+  static constexpr int NKC = (NE * NF + NT - 1) / NT;  // accumulators per lane
+  static constexpr int64_t scratch() { return IN_LDS ? 0 : 2 * buf_doubles(); }
 };
 
 template <int LA, int LB, int LC, int LD>
@@ -944,12 +974,43 @@ __global__ __launch_bounds__((Coop<LA, LB, LC, LD>::NT)) void eri_coop_kernel(Cl
   using S = Coop<LA, LB, LC, LD>;
   constexpr int NT = S::NT, NR = S::NR, NI = S::NI, NK = S::NK, NE = S::NE, NF = S::NF, NKC = S::NKC;
   constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND, NABCD = S::NABCD;
+  constexpr int NAB = S::NAB, NCD = S::NCD, NAC = S::NAC, NBD = S::NBD, NAD = S::NAD, NBC = S::NBC;
   constexpr int GSZ = NI * NK;
-  __shared__ double G[3][NR][GSZ];
+  __shared__ double lds[S::LDS_DOUBLES];
+  double *Gt = lds;                   // [3][NR][GSZ]
+  double *Dab = Gt + S::GT, *Dcd = Dab + NAB, *Dbd = Dcd + NCD, *Dac = Dbd + NBD, *Dbc = Dac + NAC, *Dad = Dbc + NBC;
+  int *tab = (int *)(Dad + NAD);      // HRR step tables
+  double *buf0 = S::IN_LDS ? lds + S::fixed_doubles() : ba.scratch + (int64_t)blockIdx.x * ba.scratch_per_wg;
+  double *buf1 = buf0 + S::buf_doubles();
 
   const int t = threadIdx.x;
-  double *buf0 = ba.scratch + (int64_t)blockIdx.x * ba.scratch_per_wg;
-  double *buf1 = buf0 + S::buf_doubles();
+  // step tables: output row -> source rows s1 (shifted) and s2, direction
+  for (int i = t; i < S::NTAB; i += NT) {
+    int lev = 0;
+    while (lev + 1 < LB + LD && S::tab_off(lev + 1) <= i) ++lev;
+    const int r = i - S::tab_off(lev);
+    int s1, s2, dir;
+    if (lev < LB) {  // bra level j: row (a in range(LA, LAB-j), b in cart(j))
+      const int j = lev + 1, nb = ncart(j), nbm = ncart(j - 1), a = r / nb, b = r % nb;
+      int bx, by, bz, La, ax, ay, az;
+      cart_decode(j, b, bx, by, bz);
+      range_decode(LA, a, La, ax, ay, az);
+      dir = bx ? 0 : (by ? 1 : 2);
+      const int bmi = cidx(bx - (dir == 0), by - (dir == 1), bz - (dir == 2));
+      s1 = range_index(LA, ax + (dir == 0), ay + (dir == 1), az + (dir == 2)) * nbm + bmi;
+      s2 = a * nbm + bmi;
+    } else {  // ket level j: column (c in range(LC, LCD-j), d in cart(j))
+      const int j = lev - LB + 1, nd = ncart(j), ndm = ncart(j - 1), c = r / nd, d = r % nd;
+      int dx, dy, dz, Lc, cx, cy, cz;
+      cart_decode(j, d, dx, dy, dz);
+      range_decode(LC, c, Lc, cx, cy, cz);
+      dir = dx ? 0 : (dy ? 1 : 2);
+      const int dmi = cidx(dx - (dir == 0), dy - (dir == 1), dz - (dir == 2));
+      s1 = range_index(LC, cx + (dir == 0), cy + (dir == 1), cz + (dir == 2)) * ndm + dmi;
+      s2 = c * ndm + dmi;
+    }
+    tab[i] = s1 | (s2 << 14) | (dir << 28);
+  }
 
   // per-lane component offsets into the G tables (fixed for the kernel)
   int offx[NKC], offy[NKC], offz[NKC];
@@ -967,12 +1028,36 @@ __global__ __launch_bounds__((Coop<LA, LB, LC, LD>::NT)) void eri_coop_kernel(Cl
       offx[k] = offy[k] = offz[k] = 0;
     }
   }
+  const double *Dm = ba.D, *Dk = ba.DK;
+  double *F = ba.F;
+  const double ks = ba.k_scale;
 
   for (int64_t q = cd.q_begin + blockIdx.x; q < cd.q_end; q += gridDim.x) {
     int32_t ib, ik;
     decode_quartet(cd, q, ib, ik);
     const PairRec bra = ba.pairs[ib];
     const PairRec ket = ba.pairs[ik];
+    // the six density blocks of the digestion (Coulomb: D(ab), D(cd); exchange: the rest)
+    for (int i = t; i < S::DB; i += NT) {
+      int k = i;
+      const double *src = Dm;
+      uint32_t r0, r1;
+      if (k < NAB) {
+        r0 = bra.bfA + gpos_rt(LA, k / NB), r1 = bra.bfB + gpos_rt(LB, k % NB);
+      } else if ((k -= NAB) < NCD) {
+        r0 = ket.bfA + gpos_rt(LC, k / ND), r1 = ket.bfB + gpos_rt(LD, k % ND);
+      } else if (src = Dk, (k -= NCD) < NBD) {
+        r0 = bra.bfB + gpos_rt(LB, k / ND), r1 = ket.bfB + gpos_rt(LD, k % ND);
+      } else if ((k -= NBD) < NAC) {
+        r0 = bra.bfA + gpos_rt(LA, k / NC), r1 = ket.bfA + gpos_rt(LC, k % NC);
+      } else if ((k -= NAC) < NBC) {
+        r0 = bra.bfB + gpos_rt(LB, k / NC), r1 = ket.bfA + gpos_rt(LC, k % NC);
+      } else {
+        k -= NBC;
+        r0 = bra.bfA + gpos_rt(LA, k / ND), r1 = ket.bfB + gpos_rt(LD, k % ND);
+      }
+      Dab[i] = src[tri(r0, r1)];
+    }
     double acc[NKC];
 #pragma unroll
     for (int k = 0; k < NKC; ++k) acc[k] = 0.0;
@@ -1003,14 +1088,15 @@ __global__ __launch_bounds__((Coop<LA, LB, LC, LD>::NT)) void eri_coop_kernel(Cl
 #pragma unroll
           for (int i = 0; i < NI; ++i)
 #pragma unroll
-            for (int k = 0; k < NK; ++k) G[dim][r][i * NK + k] = g[i][k];
+            for (int k = 0; k < NK; ++k) Gt[(dim * NR + r) * GSZ + i * NK + k] = g[i][k];
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < NKC; ++k) {
           double v = 0.0;
 #pragma unroll
-          for (int r = 0; r < NR; ++r) v += G[0][r][offx[k]] * G[1][r][offy[k]] * G[2][r][offz[k]];
+          for (int r = 0; r < NR; ++r)
+            v += Gt[r * GSZ + offx[k]] * Gt[(NR + r) * GSZ + offy[k]] * Gt[(2 * NR + r) * GSZ + offz[k]];
           acc[k] += v;
         }
         __syncthreads();
@@ -1023,58 +1109,43 @@ __global__ __launch_bounds__((Coop<LA, LB, LC, LD>::NT)) void eri_coop_kernel(Cl
     }
     __syncthreads();
 
-    // bra HRR: level j out[a in range(LA, LAB-j)][b in cart(j)][f]
+    // bra HRR: level j out[a in range(LA, LAB-j)][b in cart(j)][f] from level j-1
     double *in = buf0, *out = buf1;
+#pragma unroll
     for (int j = 1; j <= LB; ++j) {
-      const int na = nrange(LA, S::LAB - j), nb = ncart(j), nbm = ncart(j - 1);
-      const int items = na * nb * NF;
+      const int *tj = tab + S::tab_off(j - 1);
+      const int items = S::bra_rows(j) * NF;
       for (int it = t; it < items; it += NT) {
-        const int f = it % NF, rest = it / NF, b = rest % nb, a = rest / nb;
-        int bx, by, bz, La, ax, ay, az;
-        cart_decode(j, b, bx, by, bz);
-        range_decode(LA, a, La, ax, ay, az);
-        const int dir = bx ? 0 : (by ? 1 : 2);
-        const int bmi = cidx(bx - (dir == 0), by - (dir == 1), bz - (dir == 2));
-        const int api = range_index(LA, ax + (dir == 0), ay + (dir == 1), az + (dir == 2));
+        const int row = it / NF, f = it - row * NF;
+        const int e = tj[row], dir = e >> 28;
         const double abv = dir == 0 ? bra.ABx : (dir == 1 ? bra.ABy : bra.ABz);
-        out[(a * nb + b) * NF + f] = in[(api * nbm + bmi) * NF + f] + abv * in[(a * nbm + bmi) * NF + f];
+        out[it] = in[(e & 16383) * NF + f] + abv * in[((e >> 14) & 16383) * NF + f];
       }
       __syncthreads();
       double *tmp = in;
       in = out;
       out = tmp;
     }
-    // ket HRR on the inner index
+    // ket HRR on the inner index: out[ab][c in range(LC, LCD-j)][d in cart(j)]
+#pragma unroll
     for (int j = 1; j <= LD; ++j) {
-      const int nc = nrange(LC, S::LCD - j), nd = ncart(j), ndm = ncart(j - 1);
-      const int nin = nrange(LC, S::LCD - j + 1) * ndm, nout = nc * nd;
-      const int items = NA * NB * nout;
-      for (int it = t; it < items; it += NT) {
-        const int ab = it / nout, rest = it % nout, c = rest / nd, d = rest % nd;
-        int dx, dy, dz, Lc, cx, cy, cz;
-        cart_decode(j, d, dx, dy, dz);
-        range_decode(LC, c, Lc, cx, cy, cz);
-        const int dir = dx ? 0 : (dy ? 1 : 2);
-        const int dmi = cidx(dx - (dir == 0), dy - (dir == 1), dz - (dir == 2));
-        const int cpi = range_index(LC, cx + (dir == 0), cy + (dir == 1), cz + (dir == 2));
+      const int *tj = tab + S::tab_off(LB + j - 1);
+      const int nin = S::ket_rows(j - 1), nout = S::ket_rows(j);
+      for (int it = t; it < NAB * nout; it += NT) {
+        const int ab = it / nout, cdi = it - ab * nout;
+        const int e = tj[cdi], dir = e >> 28;
         const double cdv = dir == 0 ? ket.ABx : (dir == 1 ? ket.ABy : ket.ABz);
-        out[ab * nout + c * nd + d] = in[ab * nin + cpi * ndm + dmi] + cdv * in[ab * nin + c * ndm + dmi];
+        out[it] = in[ab * nin + (e & 16383)] + cdv * in[ab * nin + ((e >> 14) & 16383)];
       }
       __syncthreads();
       double *tmp = in;
       in = out;
       out = tmp;
     }
-    // scale + value cutoff, in place: v[a][b][c][d]
+    // cartesian scales and the value cutoff, in place: v[a][b][c][d]
     for (int it = t; it < NABCD; it += NT) {
-      const int d = it % ND, c = (it / ND) % NC, b = (it / (ND * NC)) % NB, a = it / (ND * NC * NB);
-      int x0, y0, z0, x1, y1, z1, x2, y2, z2, x3, y3, z3;
-      cart_decode(LA, a, x0, y0, z0);
-      cart_decode(LB, b, x1, y1, z1);
-      cart_decode(LC, c, x2, y2, z2);
-      cart_decode(LD, d, x3, y3, z3);
-      const double v = in[it] * cart_scale(x0, y0, z0) * cart_scale(x1, y1, z1) * cart_scale(x2, y2, z2) *
-                       cart_scale(x3, y3, z3);
+      const int d = it % ND, c = (it / ND) % NC, b = (it / NCD) % NB, a = it / (NB * NCD);
+      const double v = in[it] * scale_rt(LA, a) * scale_rt(LB, b) * scale_rt(LC, c) * scale_rt(LD, d);
       in[it] = fabs(v) < ba.value_cut ? 0.0 : v;
     }
     __syncthreads();
@@ -1084,110 +1155,60 @@ __global__ __launch_bounds__((Coop<LA, LB, LC, LD>::NT)) void eri_coop_kernel(Cl
     if (ket.sameAB) s *= 0.5;
     if (cd.diag && ib == ik) s *= 0.5;
     const double *v = in;
-    const double *Dm = ba.D, *Dk = ba.DK;
-    double *F = ba.F;
-    const double ks = ba.k_scale;
-    // six contracted blocks; one output element per lane iteration
-    constexpr int n1 = NA * NB, n2 = NC * ND, n3 = NA * NC, n4 = NB * ND, n5 = NA * ND, n6 = NB * NC;
-    for (int it = t; it < n1 + n2 + n3 + n4 + n5 + n6; it += NT) {
-      int x, y, z;
-      double acc2 = 0.0;
-      int64_t fi, fj;
-      double fac;
+    // six contracted blocks, one output element per lane iteration; the sums
+    // run over compile-time strides from the output's base
+    constexpr int n1 = NAB, n2 = n1 + NCD, n3 = n2 + NAC, n4 = n3 + NBD, n5 = n4 + NAD, n6 = n5 + NBC;
+    for (int it = t; it < n6; it += NT) {
+      double acc2 = 0.0, fac;
+      uint32_t fi, fj;
       if (it < n1) {  // F[ab] += 4s sum_cd v D[cd]
         const int a = it / NB, b = it % NB;
-        cart_decode(LA, a, x, y, z);
-        fi = bra.bfA + gamess_pos(x, y, z);
-        cart_decode(LB, b, x, y, z);
-        fj = bra.bfB + gamess_pos(x, y, z);
-        for (int c = 0; c < NC; ++c) {
-          cart_decode(LC, c, x, y, z);
-          const int64_t gc = ket.bfA + gamess_pos(x, y, z);
-          for (int d = 0; d < ND; ++d) {
-            cart_decode(LD, d, x, y, z);
-            acc2 += v[((a * NB + b) * NC + c) * ND + d] * Dm[tri(gc, ket.bfB + gamess_pos(x, y, z))];
-          }
-        }
-        fac = 4.0 * s;
-      } else if (it < n1 + n2) {  // F[cd] += 4s sum_ab v D[ab]
-        const int k = it - n1, c = k / ND, d = k % ND;
-        cart_decode(LC, c, x, y, z);
-        fi = ket.bfA + gamess_pos(x, y, z);
-        cart_decode(LD, d, x, y, z);
-        fj = ket.bfB + gamess_pos(x, y, z);
-        for (int a = 0; a < NA; ++a) {
-          cart_decode(LA, a, x, y, z);
-          const int64_t ga = bra.bfA + gamess_pos(x, y, z);
-          for (int b = 0; b < NB; ++b) {
-            cart_decode(LB, b, x, y, z);
-            acc2 += v[((a * NB + b) * NC + c) * ND + d] * Dm[tri(ga, bra.bfB + gamess_pos(x, y, z))];
-          }
-        }
-        fac = 4.0 * s;
-      } else if (it < n1 + n2 + n3) {  // F[ac] -= s sum_bd v D[bd]
-        const int k = it - n1 - n2, a = k / NC, c = k % NC;
-        cart_decode(LA, a, x, y, z);
-        fi = bra.bfA + gamess_pos(x, y, z);
-        cart_decode(LC, c, x, y, z);
-        fj = ket.bfA + gamess_pos(x, y, z);
-        for (int b = 0; b < NB; ++b) {
-          cart_decode(LB, b, x, y, z);
-          const int64_t gb = bra.bfB + gamess_pos(x, y, z);
-          for (int d = 0; d < ND; ++d) {
-            cart_decode(LD, d, x, y, z);
-            acc2 += v[((a * NB + b) * NC + c) * ND + d] * Dk[tri(gb, ket.bfB + gamess_pos(x, y, z))];
-          }
-        }
-        fac = -s * ks;
-      } else if (it < n1 + n2 + n3 + n4) {  // F[bd] -= s sum_ac v D[ac]
-        const int k = it - n1 - n2 - n3, b = k / ND, d = k % ND;
-        cart_decode(LB, b, x, y, z);
-        fi = bra.bfB + gamess_pos(x, y, z);
-        cart_decode(LD, d, x, y, z);
-        fj = ket.bfB + gamess_pos(x, y, z);
-        for (int a = 0; a < NA; ++a) {
-          cart_decode(LA, a, x, y, z);
-          const int64_t ga = bra.bfA + gamess_pos(x, y, z);
-          for (int c = 0; c < NC; ++c) {
-            cart_decode(LC, c, x, y, z);
-            acc2 += v[((a * NB + b) * NC + c) * ND + d] * Dk[tri(ga, ket.bfA + gamess_pos(x, y, z))];
-          }
-        }
-        fac = -s * ks;
-      } else if (it < n1 + n2 + n3 + n4 + n5) {  // F[ad] -= s sum_bc v D[bc]
-        const int k = it - n1 - n2 - n3 - n4, a = k / ND, d = k % ND;
-        cart_decode(LA, a, x, y, z);
-        fi = bra.bfA + gamess_pos(x, y, z);
-        cart_decode(LD, d, x, y, z);
-        fj = ket.bfB + gamess_pos(x, y, z);
-        for (int b = 0; b < NB; ++b) {
-          cart_decode(LB, b, x, y, z);
-          const int64_t gb = bra.bfB + gamess_pos(x, y, z);
-          for (int c = 0; c < NC; ++c) {
-            cart_decode(LC, c, x, y, z);
-            acc2 += v[((a * NB + b) * NC + c) * ND + d] * Dk[tri(gb, ket.bfA + gamess_pos(x, y, z))];
-          }
-        }
-        fac = -s * ks;
+        const double *vb = v + it * NCD;
+#pragma unroll 4
+        for (int k = 0; k < NCD; ++k) acc2 += vb[k] * Dcd[k];
+        fi = bra.bfA + gpos_rt(LA, a), fj = bra.bfB + gpos_rt(LB, b), fac = 4.0 * s;
+      } else if (it < n2) {  // F[cd] += 4s sum_ab v D[ab]
+        const int k0 = it - n1, c = k0 / ND, d = k0 % ND;
+        const double *vb = v + k0;
+#pragma unroll 4
+        for (int k = 0; k < NAB; ++k) acc2 += vb[k * NCD] * Dab[k];
+        fi = ket.bfA + gpos_rt(LC, c), fj = ket.bfB + gpos_rt(LD, d), fac = 4.0 * s;
+      } else if (it < n3) {  // F[ac] -= s sum_bd v D[bd]
+        const int k0 = it - n2, a = k0 / NC, c = k0 % NC;
+        const double *vb = v + a * NB * NCD + c * ND;
+#pragma unroll 4
+        for (int b = 0; b < NB; ++b)
+#pragma unroll 4
+          for (int d = 0; d < ND; ++d) acc2 += vb[b * NCD + d] * Dbd[b * ND + d];
+        fi = bra.bfA + gpos_rt(LA, a), fj = ket.bfA + gpos_rt(LC, c), fac = -s * ks;
+      } else if (it < n4) {  // F[bd] -= s sum_ac v D[ac]
+        const int k0 = it - n3, b = k0 / ND, d = k0 % ND;
+        const double *vb = v + b * NCD + d;
+#pragma unroll 4
+        for (int a = 0; a < NA; ++a)
+#pragma unroll 4
+          for (int c = 0; c < NC; ++c) acc2 += vb[a * NB * NCD + c * ND] * Dac[a * NC + c];
+        fi = bra.bfB + gpos_rt(LB, b), fj = ket.bfB + gpos_rt(LD, d), fac = -s * ks;
+      } else if (it < n5) {  // F[ad] -= s sum_bc v D[bc]
+        const int k0 = it - n4, a = k0 / ND, d = k0 % ND;
+        const double *vb = v + a * NB * NCD + d;
+#pragma unroll 4
+        for (int b = 0; b < NB; ++b)
+#pragma unroll 4
+          for (int c = 0; c < NC; ++c) acc2 += vb[b * NCD + c * ND] * Dbc[b * NC + c];
+        fi = bra.bfA + gpos_rt(LA, a), fj = ket.bfB + gpos_rt(LD, d), fac = -s * ks;
       } else {  // F[bc] -= s sum_ad v D[ad]
-        const int k = it - n1 - n2 - n3 - n4 - n5, b = k / NC, c = k % NC;
-        cart_decode(LB, b, x, y, z);
-        fi = bra.bfB + gamess_pos(x, y, z);
-        cart_decode(LC, c, x, y, z);
-        fj = ket.bfA + gamess_pos(x, y, z);
-        for (int a = 0; a < NA; ++a) {
-          cart_decode(LA, a, x, y, z);
-          const int64_t ga = bra.bfA + gamess_pos(x, y, z);
-          for (int d = 0; d < ND; ++d) {
-            cart_decode(LD, d, x, y, z);
-            acc2 += v[((a * NB + b) * NC + c) * ND + d] * Dk[tri(ga, ket.bfB + gamess_pos(x, y, z))];
-          }
-        }
-        fac = -s * ks;
+        const int k0 = it - n5, b = k0 / NC, c = k0 % NC;
+        const double *vb = v + b * NCD + c * ND;
+#pragma unroll 4
+        for (int a = 0; a < NA; ++a)
+#pragma unroll 4
+          for (int d = 0; d < ND; ++d) acc2 += vb[a * NB * NCD + d] * Dad[a * ND + d];
+        fi = bra.bfB + gpos_rt(LB, b), fj = ket.bfA + gpos_rt(LC, c), fac = -s * ks;
       }
       if (acc2 != 0.0) fock_add(F, fi, fj, fac * acc2);
     }
-    __syncthreads();  // scratch is reused by the next quartet
+    __syncthreads();  // buffers and density blocks are reused by the next quartet
   }
 }
 
@@ -1297,19 +1318,6 @@ __host__ __device__ constexpr int grp_buffer(int LA, int LB, int LC, int LD, int
     }
   }
   return m;
-}
-
-// component k (internal order) of a shell of angular momentum L -> GAMESS
-// position and cartesian scale factor, without loops
-__device__ __forceinline__ int gpos_rt(int L, int k) {
-  if (L <= 1) return k;
-  if (L == 2) return (int)((0x251430u >> (4 * k)) & 15u);
-  return (int)((0x2861795430ull >> (4 * k)) & 15ull);
-}
-__device__ __forceinline__ double scale_rt(int L, int k) {
-  if (L == 2) return (k == 1 || k == 2 || k == 4) ? 1.7320508075688772935 : 1.0;
-  if (L == 3) return (k == 0 || k == 6 || k == 9) ? 1.0 : (k == 4 ? 3.8729833462074168852 : 2.2360679774997896964);
-  return 1.0;
 }
 
 // bra HRR level J for the columns f of this lane: out[a in range(LA, LAB-J)][b in
