@@ -83,7 +83,7 @@ struct DevBuf {
 struct ClassState {
   bool active = false;
   bool row_ok = false;  // a row-tiled kernel exists and fits in LDS
-  int variant = -1;     // digestion chosen for the class: 0 direct, 1 row-tiled (-1: not yet)
+  int variant = -1;     // form chosen for the class: 0 direct, 1 row-tiled, 3 cooperative (-1: not yet)
   int fp32 = -1;        // mixed form: 1 FP32 allowed, 0 not (-1: not yet chosen for the current tau)
   float ms = -1.f;      // device time of the class's launch, measured at the first build
   int64_t nquart = 0, nprimq = 0;
@@ -396,6 +396,7 @@ int liberi_c_get_stats(liberi_ctx *h, liberi_stats *out) {
     out->class_variant[c] = !cs_.active || cs_.variant < 0       ? -1
                             : h->last_fused && class_fused_nq(c) > 0 ? 4
                             : row                                 ? 1 + cs_.dev.transposed
+                            : cs_.variant == 3                    ? 3
                                                                   : 0;
     out->class_fp32[c] = cs_.active && h->fp32_tau > 0.0 && cs_.fp32 == 1;
     out->class_tile[c] = cs_.active && row && cs_.tile >= 0 && !(h->last_fused && class_fused_nq(c) > 0)
@@ -992,6 +993,31 @@ static int choose_variants(liberi_ctx *h, const BuildArgs &ba, hipStream_t st) {
     if (!cs_.active || cs_.variant >= 0) continue;
     const int dflt = (cs_.row_ok && class_kernel_kind(c) == 0) ? 1 : 0;
     const int64_t nq = cs_.dev.q_end - cs_.dev.q_begin;
+    if (class_has_coop(c)) {
+      // group class with a cooperative form (variant 3): one launch of each
+      // is cheap at any size, so it is timed without a size threshold
+      cs_.variant = 0;
+      cs_.dev.coop = 0;
+      if (!h->autotune || (h->only_class >= 0 && c != h->only_class)) continue;
+      if (const int prev = tuned_choice(c, nq, -3.0); prev >= 0) {
+        cs_.variant = prev;
+        cs_.dev.coop = prev == 3;
+        continue;
+      }
+      float best[2] = {1e30f, 1e30f};
+      for (int rep = 0; rep < 4; ++rep) {
+        const int v = rep & 1;
+        float ms = 0.f;
+        cs_.dev.coop = v;
+        if (int rc = time_class(h, ba, st, c, false, false, ms)) return rc;
+        best[v] = std::min(best[v], ms);
+      }
+      cs_.variant = best[1] < best[0] ? 3 : 0;
+      cs_.dev.coop = cs_.variant == 3;
+      cs_.ms = std::min(best[0], best[1]);
+      set_tuned_choice(c, nq, -3.0, cs_.variant);
+      continue;
+    }
     if (!h->autotune || h->row_tiles != 1 || !cs_.row_ok || nq < kTuneMinQuartets || cs_.dev.t_end <= cs_.dev.t_begin ||
         (h->only_class >= 0 && c != h->only_class)) {
       cs_.variant = dflt;

@@ -973,7 +973,7 @@ template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__((Coop<LA, LB, LC, LD>::NT)) void eri_coop_kernel(ClassDev cd, BuildArgs ba) {
   using S = Coop<LA, LB, LC, LD>;
   constexpr int NT = S::NT, NR = S::NR, NI = S::NI, NK = S::NK, NE = S::NE, NF = S::NF, NKC = S::NKC;
-  constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND, NABCD = S::NABCD;
+  constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND;
   constexpr int NAB = S::NAB, NCD = S::NCD, NAC = S::NAC, NBD = S::NBD, NAD = S::NAD, NBC = S::NBC;
   constexpr int GSZ = NI * NK;
   __shared__ double lds[S::LDS_DOUBLES];
@@ -1032,9 +1032,18 @@ __global__ __launch_bounds__((Coop<LA, LB, LC, LD>::NT)) void eri_coop_kernel(Cl
   double *F = ba.F;
   const double ks = ba.k_scale;
 
-  for (int64_t q = cd.q_begin + blockIdx.x; q < cd.q_end; q += gridDim.x) {
+  // a contiguous chunk of the class's quartets per workgroup: the segment is
+  // searched once and then followed, and consecutive quartets share the bra
+  // pair (the cached bra records)
+  const int64_t nq = cd.q_end - cd.q_begin;
+  const int64_t per = (nq + gridDim.x - 1) / gridDim.x;
+  const int64_t q0 = cd.q_begin + (int64_t)blockIdx.x * per;
+  const int64_t q1 = q0 + per < cd.q_end ? q0 + per : cd.q_end;
+  int sg = q0 < q1 ? find_segment(cd, q0, 0, cd.nseg) : 0;
+  for (int64_t q = q0; q < q1; ++q) {
+    while (sg + 1 < cd.nseg && cd.seg_start[sg + 1] <= q) ++sg;
     int32_t ib, ik;
-    decode_quartet(cd, q, ib, ik);
+    quartet_of_segment(cd, sg, q, ib, ik);
     const PairRec bra = ba.pairs[ib];
     const PairRec ket = ba.pairs[ik];
     // the six density blocks of the digestion (Coulomb: D(ab), D(cd); exchange: the rest)
@@ -1102,10 +1111,17 @@ __global__ __launch_bounds__((Coop<LA, LB, LC, LD>::NT)) void eri_coop_kernel(Cl
         __syncthreads();
       }
     }
+    // cartesian scales and the value cutoff, applied by the last write of the
+    // HRR (whose output index is v's [a][b][c][d])
+    auto final_value = [&](int it, double x) -> double {
+      const int d = it % ND, c = (it / ND) % NC, b = (it / NCD) % NB, a = it / (NB * NCD);
+      const double v = x * scale_rt(LA, a) * scale_rt(LB, b) * scale_rt(LC, c) * scale_rt(LD, d);
+      return fabs(v) < ba.value_cut ? 0.0 : v;
+    };
 #pragma unroll
     for (int k = 0; k < NKC; ++k) {
       const int c = t + NT * k;
-      if (c < NE * NF) buf0[c] = acc[k];
+      if (c < NE * NF) buf0[c] = LB == 0 && LD == 0 ? final_value(c, acc[k]) : acc[k];
     }
     __syncthreads();
 
@@ -1119,7 +1135,8 @@ __global__ __launch_bounds__((Coop<LA, LB, LC, LD>::NT)) void eri_coop_kernel(Cl
         const int row = it / NF, f = it - row * NF;
         const int e = tj[row], dir = e >> 28;
         const double abv = dir == 0 ? bra.ABx : (dir == 1 ? bra.ABy : bra.ABz);
-        out[it] = in[(e & 16383) * NF + f] + abv * in[((e >> 14) & 16383) * NF + f];
+        const double x = in[(e & 16383) * NF + f] + abv * in[((e >> 14) & 16383) * NF + f];
+        out[it] = j == LB && LD == 0 ? final_value(it, x) : x;
       }
       __syncthreads();
       double *tmp = in;
@@ -1135,20 +1152,14 @@ __global__ __launch_bounds__((Coop<LA, LB, LC, LD>::NT)) void eri_coop_kernel(Cl
         const int ab = it / nout, cdi = it - ab * nout;
         const int e = tj[cdi], dir = e >> 28;
         const double cdv = dir == 0 ? ket.ABx : (dir == 1 ? ket.ABy : ket.ABz);
-        out[it] = in[ab * nin + (e & 16383)] + cdv * in[ab * nin + ((e >> 14) & 16383)];
+        const double x = in[ab * nin + (e & 16383)] + cdv * in[ab * nin + ((e >> 14) & 16383)];
+        out[it] = j == LD ? final_value(it, x) : x;
       }
       __syncthreads();
       double *tmp = in;
       in = out;
       out = tmp;
     }
-    // cartesian scales and the value cutoff, in place: v[a][b][c][d]
-    for (int it = t; it < NABCD; it += NT) {
-      const int d = it % ND, c = (it / ND) % NC, b = (it / NCD) % NB, a = it / (NB * NCD);
-      const double v = in[it] * scale_rt(LA, a) * scale_rt(LB, b) * scale_rt(LC, c) * scale_rt(LD, d);
-      in[it] = fabs(v) < ba.value_cut ? 0.0 : v;
-    }
-    __syncthreads();
 
     double s = 1.0;
     if (bra.sameAB) s *= 0.5;
@@ -2377,6 +2388,14 @@ int64_t row_lds(const int32_t *ncol, double xswitch) {
     return -1;
 }
 
+// group classes with an f shell also get the cooperative form when its
+// buffers fit the LDS; which one runs is timed per class
+template <int LA, int LB, int LC, int LD>
+constexpr bool has_coop_alt() {
+  return !use_small<LA, LB, LC, LD>() && Grp<LA, LB, LC, LD>::fits && Coop<LA, LB, LC, LD>::IN_LDS &&
+         (LA == 3 || LB == 3 || LC == 3 || LD == 3);
+}
+
 template <int LA, int LB, int LC, int LD, bool TR>
 static int launch_row(const ClassDev &cd, const BuildArgs &ba, hipStream_t st) {
   const int64_t lds = row_lds<LA, LB, LC, LD, TR>(ba.ncol, cd.xswitch);
@@ -2406,6 +2425,16 @@ static int launch_one(const ClassDev &cd, const BuildArgs &ba, hipStream_t st, i
     }
   } else if constexpr (Grp<LA, LB, LC, LD>::fits) {
     if (cd.row_mode) return hipErrorInvalidValue;
+    if (cd.coop) {
+      if constexpr (has_coop_alt<LA, LB, LC, LD>()) {
+        using S = Coop<LA, LB, LC, LD>;
+        const int64_t g = n < nwg ? n : nwg;
+        hipLaunchKernelGGL((eri_coop_kernel<LA, LB, LC, LD>), dim3((unsigned)g), dim3(S::NT), 0, st, cd, ba);
+        return hipGetLastError();
+      } else {
+        return hipErrorInvalidValue;
+      }
+    }
     constexpr int NQ = Grp<LA, LB, LC, LD>::NQ;
     const int64_t need = (n + NQ - 1) / NQ;
     const int64_t g = need < kGroupWorkgroups ? need : kGroupWorkgroups;
@@ -2630,6 +2659,18 @@ int class_kernel_kind(int c) {
 #undef X
     default:
       return -1;
+  }
+}
+
+int class_has_coop(int c) {
+  switch (c) {
+#define X(id, a, b, cc, d) \
+  case id:                 \
+    return has_coop_alt<a, b, cc, d>() ? 1 : 0;
+    LIBERI_CLASS_LIST(X)
+#undef X
+    default:
+      return 0;
   }
 }
 

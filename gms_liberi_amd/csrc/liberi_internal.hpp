@@ -99,6 +99,8 @@ struct ClassDev {
   // 1: the mixed form may evaluate this class's quartets in FP32 (chosen per
   // class at the first build under a given switch tau)
   int32_t fp32;
+  // 1: a group class with a cooperative form (class_has_coop) runs that form
+  int32_t coop;
 };
 
 struct BuildArgs {
@@ -137,6 +139,9 @@ int64_t class_row_lds(int c, int transposed, const int32_t *ncol, double xswitch
 // kernel shape of class c: 0 lane per quartet, 1 workgroup group per
 // quartet, 2 cooperative workgroup per quartet
 int class_kernel_kind(int c);
+// 1 when group class c also has the cooperative workgroup-per-quartet form
+// with its buffers in LDS (classes with an f shell), selected by ClassDev::coop
+int class_has_coop(int c);
 
 // Fused group launch: one work item = quartets [c0, c1) of class cls (relative
 // to the class's share, c0 a multiple of its quartets per round)

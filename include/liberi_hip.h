@@ -190,8 +190,10 @@ typedef struct {
   int64_t prim_pairs_dropped;           /* primitive pairs removed by LIBERI_OPT_PRIM_CUTOFF */
   int32_t class_variant[LIBERI_NCLASS]; /* kernel form used per class in the last build: 0 its own
                                            launch with direct atomics, 1 row tiles over bra pairs,
-                                           2 row tiles over ket pairs, 4 inside the fused group launch
-                                           (LIBERI_OPT_FUSED); -1 class inactive or not yet built */
+                                           2 row tiles over ket pairs, 3 the cooperative workgroup-per-
+                                           quartet form (group classes with an f shell, timed against
+                                           the group form at the first build), 4 inside the fused group
+                                           launch (LIBERI_OPT_FUSED); -1 class inactive or not yet built */
   int32_t class_fp32[LIBERI_NCLASS];    /* 1: the mixed form (LIBERI_OPT_FP32_TAU > 0) evaluates this
                                            class's qualifying quartets in FP32, chosen per class from
                                            measured time at the first build under that tau; 0: FP64 */

@@ -245,7 +245,7 @@ static void md_quartet(const shell_t *A, const shell_t *B, const shell_t *C, con
   const int na = NCART[A->l], nb = NCART[B->l], nc = NCART[C->l], nd = NCART[D->l];
   const int nab = na * nb, ncd = nc * nd, n = nab * ncd;
   const int L = A->l + B->l + C->l + D->l, n1 = L + 1;
-  const int lab = A->l + B->l, lcd = C->l + D->l;
+  const int lab = A->l + B->l;
   memset(out, 0, sizeof(double) * n);
   double ab2 = 0, cd2 = 0;
   for (int x = 0; x < 3; ++x) {
