@@ -129,6 +129,7 @@ struct liberi_ctx {
   int64_t fused_n = 0;
   int fused_nwg = 0;
   bool fused = false;      // LIBERI_OPT_FUSED
+  int row_orient = 0;      // LIBERI_OPT_ROW_ORIENT
   int tile_len = -1;       // LIBERI_OPT_TILE_LEN as an index into kTileLens; -1 chosen per class
   bool last_fused = false; // the last build ran the fused launch
   DevBuf tune_F;        // autotuner's scratch Fock matrix
@@ -343,6 +344,10 @@ int liberi_c_set_option(liberi_ctx *h, int32_t opt, double v) {
       h->rank_split = (int)v;
       break;
     case LIBERI_OPT_FUSED: h->fused = v != 0.0; break;
+    case LIBERI_OPT_ROW_ORIENT:
+      if (v != 0.0 && v != 1.0 && v != 2.0) return fail(LIBERI_EARG, "liberi_c_set_option: ROW_ORIENT must be 0, 1 or 2");
+      h->row_orient = (int)v;
+      break;
     case LIBERI_OPT_TILE_LEN: {
       int k = -1;
       for (int i = 0; i < kNumTileLens; ++i)
@@ -380,6 +385,7 @@ int liberi_c_get_option(liberi_ctx *h, int32_t opt, double *v) {
     case LIBERI_OPT_MAX_DEVICE_BYTES: *v = h->max_device_bytes; break;
     case LIBERI_OPT_RANK_SPLIT: *v = h->rank_split; break;
     case LIBERI_OPT_FUSED: *v = h->fused; break;
+    case LIBERI_OPT_ROW_ORIENT: *v = h->row_orient; break;
     case LIBERI_OPT_TILE_LEN: *v = h->tile_len < 0 ? 0.0 : (double)kTileLens[h->tile_len]; break;
     default: return fail(LIBERI_EARG, "liberi_c_get_option: unknown option " + std::to_string(opt));
   }
@@ -726,8 +732,10 @@ static int setup_basis(liberi_ctx *h, int32_t nsh, int32_t natoms, int32_t num_b
       const int nket = ncart_host(kPairL[def.ket][0]) * ncart_host(kPairL[def.ket][1]);
       const double xsw = h->exact_large_t ? class_rys_xmax(c) : reference_switch(c);
       const int64_t l0 = class_row_lds(c, 0, h->ncol, xsw), l1 = diag ? -1 : class_row_lds(c, 1, h->ncol, xsw);
-      const bool fit0 = l0 >= 0 && l0 <= kRowLdsBytes, fit1 = l1 >= 0 && l1 <= kRowLdsBytes;
+      const bool fit0 = l0 >= 0, fit1 = l1 >= 0;  // (-1: no row form, or its LDS would not fit)
       transposed = fit1 && (!fit0 || nket > nbra);
+      if (h->row_orient == 1 && fit0) transposed = false;
+      if (h->row_orient == 2 && fit1) transposed = true;
       cs_.row_ok = fit0 || fit1;
     }
     if (transposed) kb.swap(kk);

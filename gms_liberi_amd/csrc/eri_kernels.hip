@@ -893,6 +893,524 @@ __global__ __launch_bounds__(256) void eri_row_kernel(ClassDev cd, BuildArgs ba)
   }
 }
 
+// ====================================================== wave-split row path
+// The row-tiled form for classes whose (e0|f0) block is too large for the
+// registers of one lane: S parts share each quartet, part p accumulating one
+// slice of the block.  The slices are chosen so that each part's HRR stays
+// inside its slice:
+//  * columns (LD = 0, no ket HRR): a run of the columns f; the bra HRR runs
+//    column by column;
+//  * rows (LB = 0, no bra HRR): a run of the rows e; the ket HRR runs row by
+//    row;
+//  * ket cone (LD = 1): one component c of shell C and the four columns
+//    c, c + 1x, c + 1y, c + 1z its ket HRR reads; the ket step
+//    (e|c,1i) = (e|c+1i) + CD_i (e|c) runs first, then the bra HRR on the
+//    three columns (c, d).
+// Every part repeats the Rys rule and the 2-D recursion of each primitive
+// quartet (the compiler drops the table elements its slice never reads) and
+// keeps its slice, its HRR and its part of the digestion in registers: no
+// LDS traffic in the primitive loop.  The exchange blocks land in the
+// workgroup's LDS rows as in eri_row_kernel; a block summed over the split
+// index gets one LDS add per part.  A wave runs S / W of the parts one after
+// the other over its quartets, so that W waves serve one run of 64 quartets.
+
+enum SplitSide { SPLIT_NONE = 0, SPLIT_COLS = 1, SPLIT_ROWS = 2, SPLIT_CONE = 3 };
+
+// largest HRR intermediate of a slice: NEP rows e and NFP columns f of
+// (e0|f0), bra levels on the NFP columns, ket levels on the slice's rows ab
+__host__ __device__ constexpr int split_buffer(int LA, int LB, int LC, int LD, int NEP, int NFP) {
+  int m = NEP * NFP;
+  for (int j = 1; j <= LB; ++j) {
+    const int s = nrange(LA, LA + LB - j) * ncart(j) * NFP;
+    m = s > m ? s : m;
+  }
+  const int nab = LB == 0 ? NEP : ncart(LA) * ncart(LB);
+  for (int j = 1; j <= LD; ++j) {
+    const int s = nab * nrange(LC, LC + LD - j) * ncart(j);
+    m = s > m ? s : m;
+  }
+  return m;
+}
+
+// Parts S and waves per run W of the split form of a class (S = 0: none).
+// Chosen so that the slice, the tables and the HRR levels of a part fit the
+// registers of two waves per SIMD (8-wave workgroups) or three (12 waves).
+template <int LA, int LB, int LC, int LD>
+constexpr int split_parts() {
+  if constexpr (LA == 2 && LB == 1 && LC == 2 && LD == 0) return 2;  // (dp|ds): 3 of the 6 d columns
+  if constexpr (LA == 2 && LB == 2 && LC == 2 && LD == 0) return 6;  // (dd|ds): one d column
+  if constexpr (LA == 1 && LB == 0 && LC == 2 && LD == 2) return 3;  // (ps|dd): one p row
+  if constexpr (LA == 2 && LB == 1 && LC == 1 && LD == 1) return 3;  // (dp|pp): cone of one p of C
+  if constexpr (LA == 2 && LB == 1 && LC == 2 && LD == 1) return 6;  // (dp|dp): cone of one d of C
+  return 0;
+}
+template <int LA, int LB, int LC, int LD>
+constexpr int split_waves() {
+  constexpr int S = split_parts<LA, LB, LC, LD>();
+  if constexpr (LA == 2 && LB == 1 && LC == 1 && LD == 1) return 1;  // (dp|pp): the 3 parts in turn
+  if constexpr (LA == 2 && LB == 1 && LC == 2 && LD == 1) return 2;  // (dp|dp): 3 parts per wave
+  return S;
+}
+
+template <int LA, int LB, int LC, int LD>
+struct SplitShape {
+  static constexpr int S = split_parts<LA, LB, LC, LD>();
+  static constexpr int W = split_waves<LA, LB, LC, LD>();
+  static constexpr int NE = nrange(LA, LA + LB), NF = nrange(LC, LC + LD);
+  static constexpr int SIDE = S == 0 ? SPLIT_NONE
+                              : LD == 0 ? SPLIT_COLS
+                              : LB == 0 ? SPLIT_ROWS
+                              : LD == 1 ? SPLIT_CONE
+                                        : SPLIT_NONE;
+  static constexpr bool exists = SIDE != SPLIT_NONE && W > 0 && S % W == 0 &&
+                                 (SIDE == SPLIT_COLS   ? NF % S == 0
+                                  : SIDE == SPLIT_ROWS ? NE % S == 0
+                                                       : S == ncart(LC));
+  // rows e and columns f of (e0|f0) one part accumulates
+  static constexpr int NEP = SIDE == SPLIT_ROWS ? NE / S : NE;
+  static constexpr int NFP = SIDE == SPLIT_COLS ? NF / S : (SIDE == SPLIT_CONE ? 4 : NF);
+  // columns of the bra HRR (the part's columns, or its three (c, d) after the ket step)
+  static constexpr int NFB = SIDE == SPLIT_CONE ? 3 : NFP;
+  // waves per workgroup: W per run of 64 quartets, NBAT runs; 8 waves when W
+  // divides it, else 12 (two or three waves per SIMD)
+  static constexpr int NBAT = !exists ? 1 : (8 % W == 0 ? 8 / W : 12 / W);
+  static constexpr int NT = 64 * W * NBAT;
+  static constexpr int NR = (LA + LB + LC + LD) / 2 + 1;
+  // slice of the block after the HRR: rows a (all but in the rows split),
+  // columns c (one per part in the cone, all in the rows split)
+  static constexpr int NAP = SIDE == SPLIT_ROWS ? NEP : ncart(LA);
+  static constexpr int NCP = SIDE == SPLIT_COLS ? NFP : (SIDE == SPLIT_CONE ? 1 : ncart(LC));
+  static constexpr int NB = ncart(LB), ND = ncart(LD);
+  static constexpr int NV = NAP * NB * NCP * ND;
+  static constexpr int NBUF = SIDE == SPLIT_CONE ? split_buffer(LA, LB, LC, 0, NEP, 3) : split_buffer(LA, LB, LC, LD, NEP, NFP);
+  // first component of the slice in shell A (rows split) or C (columns, cone)
+  template <int PART>
+  static constexpr int a0() { return SIDE == SPLIT_ROWS ? PART * NEP : 0; }
+  template <int PART>
+  static constexpr int c0() { return SIDE == SPLIT_COLS ? PART * NFP : (SIDE == SPLIT_CONE ? PART : 0); }
+};
+
+// column j of part PART: its index in range(LC, LC + LD)
+template <int LA, int LB, int LC, int LD, int PART>
+constexpr int split_col(int j) {
+  using P = SplitShape<LA, LB, LC, LD>;
+  if (P::SIDE == SPLIT_COLS) return PART * P::NFP + j;
+  if (P::SIDE != SPLIT_CONE) return j;
+  constexpr Range<LC, LC> rc{};
+  if (j == 0) return PART;
+  const int i = j - 1;
+  return ncart(LC) + cidx(rc.lx[PART] + (i == 0), rc.ly[PART] + (i == 1), rc.lz[PART] + (i == 2));
+}
+
+// One contracted quartet, slice PART: the part's rows and columns of (e0|f0)
+// over the primitive loop, then the HRR of the slice, cartesian scales and
+// the value cutoff.  Result v[((a*NB + b)*NCP + c)*ND + d] with a, c
+// relative to the slice's first component.
+template <int LA, int LB, int LC, int LD, int PART, bool KET_OUTER>
+__device__ __forceinline__ void split_integrals(const PairRec &bra, const PairRec &ket, const PrimRec *prims,
+                                                double xswitch, double cut,
+                                                double (&v)[SplitShape<LA, LB, LC, LD>::NV], const double *tab,
+                                                int istride) {
+  using P = SplitShape<LA, LB, LC, LD>;
+  constexpr int NR = P::NR, NI = LA + LB + 1, NK = LC + LD + 1;
+  constexpr int NEP = P::NEP, NFP = P::NFP, NBUF = P::NBUF;
+  constexpr int E0 = P::template a0<PART>();
+  constexpr Range<LA, LA + LB> re{};
+  constexpr Range<LC, LC + LD> rf{};
+
+  double acc[NBUF > NEP * NFP ? NBUF : NEP * NFP];
+#pragma unroll
+  for (int i = 0; i < NEP * NFP; ++i) acc[i] = 0.0;
+
+  auto prim = [&](const PrimRec &Pp, const PrimRec &Q) {
+    const double rs = nr_rsqrt(Pp.p + Q.p), ipq = rs * rs;
+    const double pqx = Pp.Px - Q.Px, pqy = Pp.Py - Q.Py, pqz = Pp.Pz - Q.Pz;
+    const double X = Pp.p * Q.p * ipq * (pqx * pqx + pqy * pqy + pqz * pqz);
+    const double pref = kTwoPi52 * Pp.K * Q.K * rs;
+    double u[NR], w[NR];
+    rys_rule<NR>(X, xswitch, u, w, tab, istride);
+    const double hp = 0.5 * nr_rcp(Pp.p), hq = 0.5 * nr_rcp(Q.p);
+    const double qi = Q.p * ipq, pi = Pp.p * ipq;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const double t2 = u[r];
+      const double b00 = 0.5 * t2 * ipq;
+      const double fa = qi * t2, fc = pi * t2;
+      const double b10 = hp - hp * fa, b01 = hq - hq * fc;
+      double gx[NI][NK], gy[NI][NK], gz[NI][NK];
+      rys_2d<NI, NK>(gx, Pp.PAx - fa * pqx, Q.PAx + fc * pqx, b10, b01, b00, 1.0);
+      rys_2d<NI, NK>(gy, Pp.PAy - fa * pqy, Q.PAy + fc * pqy, b10, b01, b00, 1.0);
+      rys_2d<NI, NK>(gz, Pp.PAz - fa * pqz, Q.PAz + fc * pqz, b10, b01, b00, w[r] * pref);
+#pragma unroll
+      for (int e = 0; e < NEP; ++e)
+#pragma unroll
+        for (int f = 0; f < NFP; ++f) {
+          const int ee = E0 + e, ff = split_col<LA, LB, LC, LD, PART>(f);
+          acc[e * NFP + f] += gx[re.lx[ee]][rf.lx[ff]] * gy[re.ly[ee]][rf.ly[ff]] * gz[re.lz[ee]][rf.lz[ff]];
+        }
+    }
+  };
+  if constexpr (KET_OUTER) {
+    for (int iq = 0; iq < ket.nprim; ++iq) {
+      const PrimRec Q = prims[ket.prim_off + iq];
+      for (int ip = 0; ip < bra.nprim; ++ip) prim(prims[bra.prim_off + ip], Q);
+    }
+  } else {
+    for (int ip = 0; ip < bra.nprim; ++ip) {
+      const PrimRec Pp = prims[bra.prim_off + ip];
+      for (int iq = 0; iq < ket.nprim; ++iq) prim(Pp, prims[ket.prim_off + iq]);
+    }
+  }
+
+  // HRR of the slice
+  constexpr int NABP = LB == 0 ? NEP : ncart(LA) * ncart(LB);
+  double b1[NBUF];
+  // where the last HRR level left the slice
+  constexpr bool in_b1 =
+      P::SIDE == SPLIT_CONE ? LB % 2 == 0 : ((LB % 2) != 0) != ((LD % 2) != 0);
+  if constexpr (P::SIDE == SPLIT_CONE) {
+    // ket step into b1 as [e][d], then the bra levels on its three columns
+#pragma unroll
+    for (int e = 0; e < NEP; ++e) {
+      b1[e * 3 + 0] = acc[e * 4 + 1] + ket.ABx * acc[e * 4];
+      b1[e * 3 + 1] = acc[e * 4 + 2] + ket.ABy * acc[e * 4];
+      b1[e * 3 + 2] = acc[e * 4 + 3] + ket.ABz * acc[e * 4];
+    }
+    hrr_bra<LA, LB, 3>(b1, acc, bra.ABx, bra.ABy, bra.ABz);
+  } else {
+    if constexpr (LB > 0) hrr_bra<LA, LB, NFP>(acc, b1, bra.ABx, bra.ABy, bra.ABz);
+    if constexpr (LD > 0) {
+      if constexpr (LB % 2)
+        hrr_ket<LC, LD, NABP>(b1, acc, ket.ABx, ket.ABy, ket.ABz);
+      else
+        hrr_ket<LC, LD, NABP>(acc, b1, ket.ABx, ket.ABy, ket.ABz);
+    }
+  }
+
+  constexpr int A0 = P::template a0<PART>(), C0 = P::template c0<PART>();
+  constexpr int NAP = P::NAP, NB = P::NB, NCP = P::NCP, ND = P::ND;
+  constexpr Range<LA, LA> ra{};
+  constexpr Range<LB, LB> rb{};
+  constexpr Range<LC, LC> rc{};
+  constexpr Range<LD, LD> rd{};
+#pragma unroll
+  for (int a = 0; a < NAP; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int c = 0; c < NCP; ++c)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+          const int aa = A0 + a, cc = C0 + c;
+          const double sc = cart_scale(ra.lx[aa], ra.ly[aa], ra.lz[aa]) * cart_scale(rb.lx[b], rb.ly[b], rb.lz[b]) *
+                            cart_scale(rc.lx[cc], rc.ly[cc], rc.lz[cc]) * cart_scale(rd.lx[d], rd.ly[d], rd.lz[d]);
+          const int idx = ((a * NB + b) * NCP + c) * ND + d;
+          const double x = (in_b1 ? b1[idx] : acc[idx]) * sc;
+          v[idx] = fabs(x) < cut ? 0.0 : x;
+        }
+}
+
+// x plus the value of lane CTRL-shifted x (DPP), lanes outside the row or
+// masked off adding 0
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_add(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, ROW_MASK, 0xf, false);
+  return x + __hiloint2double(hi, lo);
+}
+// Sum over the 64 lanes of a wave, complete in lane 63: prefix doubling
+// inside each row of 16 lanes (row_shr 1, 2, 4, 8), then the row sums
+// carried over by row_bcast:15 and row_bcast:31
+__device__ __forceinline__ double wave_sum_dpp(double x) {
+  x = dpp_add<0x111, 0xf>(x);
+  x = dpp_add<0x112, 0xf>(x);
+  x = dpp_add<0x114, 0xf>(x);
+  x = dpp_add<0x118, 0xf>(x);
+  x = dpp_add<0x142, 0xa>(x);
+  x = dpp_add<0x143, 0xc>(x);
+  return x;
+}
+
+constexpr int64_t kSplitLdsBytes = 160 * 1024;
+
+// Quartets of one part (slice PART) of a wave-split row tile: the run of 64
+// quartets qb + bat*64 + lane of every step of NBAT runs.
+template <int LA, int LB, int LC, int LD, bool TR, int PART>
+__device__ __forceinline__ void split_part(const ClassDev &cd, const BuildArgs &ba, const PairRec &fix, int32_t ifix,
+                                           int64_t q0, int64_t q1, int seg0, int bat, double *jfix_s,
+                                           const double *dfix_s, double *rows, int ncols, int off2,
+                                           const double *tab) {
+  using P = SplitShape<LA, LB, LC, LD>;
+  using R = RowShape<LA, LB, LC, LD, TR>;
+  constexpr int NA = ncart(LA), NB = P::NB, NC = ncart(LC), ND = P::ND;
+  constexpr int NAP = P::NAP, NCP = P::NCP;
+  constexpr int A0 = P::template a0<PART>(), C0 = P::template c0<PART>();
+  constexpr Range<LA, LA> ra{};
+  constexpr Range<LB, LB> rb{};
+  constexpr Range<LC, LC> rc{};
+  constexpr Range<LD, LD> rd{};
+  // the moving pair's shells: M1 (its slice when it is the split one), M2
+  constexpr int NM1 = TR ? NAP : NCP, M10 = TR ? A0 : C0, NM2 = TR ? NB : ND;
+  constexpr Range<R::M1, R::M1> rm1{};
+  constexpr Range<R::M2, R::M2> rm2{};
+  // this part's share of the fixed pair's Coulomb block
+  constexpr int NJ = TR ? NCP * ND : NAP * NB;
+  const double *Dm = ba.D, *Dk = ba.DK;
+  double *F = ba.F;
+  const double sfix = fix.sameAB ? 0.5 : 1.0;
+  const int lane = threadIdx.x & 63;
+  int sg = seg0;
+  for (int64_t qb = q0; qb < q1; qb += P::NBAT * 64) {
+    const int64_t q = qb + bat * 64 + lane;
+    double jf[NJ];
+#pragma unroll
+    for (int k = 0; k < NJ; ++k) jf[k] = 0.0;
+    if (q < q1) {
+      while (sg + 1 < cd.nseg && cd.seg_start[sg + 1] <= q) ++sg;
+      const int32_t imov = cd.ket_list[cd.seg_ket0[sg] + (int32_t)(q - cd.seg_start[sg])];
+      const PairRec mov = ba.pairs[imov];
+      const PairRec &bra = TR ? mov : fix;
+      const PairRec &ket = TR ? fix : mov;
+      double v[P::NV];
+      split_integrals<LA, LB, LC, LD, PART, !TR>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, v, tab,
+                                                rys_lds_stride(P::NR));
+      double s = sfix;
+      if (mov.sameAB) s *= 0.5;
+      if (cd.diag && ifix == imov) s *= 0.5;
+      int ga[NAP], gb[NB], gc[NCP], gd[ND];
+#pragma unroll
+      for (int a = 0; a < NAP; ++a) ga[a] = bra.bfA + gamess_pos(ra.lx[A0 + a], ra.ly[A0 + a], ra.lz[A0 + a]);
+#pragma unroll
+      for (int b = 0; b < NB; ++b) gb[b] = bra.bfB + gamess_pos(rb.lx[b], rb.ly[b], rb.lz[b]);
+#pragma unroll
+      for (int c = 0; c < NCP; ++c) gc[c] = ket.bfA + gamess_pos(rc.lx[C0 + c], rc.ly[C0 + c], rc.lz[C0 + c]);
+#pragma unroll
+      for (int d = 0; d < ND; ++d) gd[d] = ket.bfB + gamess_pos(rd.lx[d], rd.ly[d], rd.lz[d]);
+      int cm1[NM1], cm2[NM2];
+#pragma unroll
+      for (int i = 0; i < NM1; ++i) cm1[i] = mov.cA + gamess_pos(rm1.lx[M10 + i], rm1.ly[M10 + i], rm1.lz[M10 + i]);
+#pragma unroll
+      for (int i = 0; i < NM2; ++i) cm2[i] = off2 + mov.cB + gamess_pos(rm2.lx[i], rm2.ly[i], rm2.lz[i]);
+      auto V = [&](int a, int b, int c, int d) -> double { return v[((a * NB + b) * NCP + c) * ND + d]; };
+
+      // Coulomb: the fixed pair's block summed over the wave, the moving pair's to F
+      if constexpr (!TR) {
+        double dcd[NCP * ND];
+#pragma unroll
+        for (int c = 0; c < NCP; ++c)
+#pragma unroll
+          for (int d = 0; d < ND; ++d) dcd[c * ND + d] = Dm[tri(gc[c], gd[d])];
+#pragma unroll
+        for (int a = 0; a < NAP; ++a)
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            double t = 0.0;
+#pragma unroll
+            for (int c = 0; c < NCP; ++c)
+#pragma unroll
+              for (int d = 0; d < ND; ++d) t += V(a, b, c, d) * dcd[c * ND + d];
+            jf[a * NB + b] = 4.0 * s * t;
+          }
+#pragma unroll
+        for (int c = 0; c < NCP; ++c)
+#pragma unroll
+          for (int d = 0; d < ND; ++d) {
+            double t = 0.0;
+#pragma unroll
+            for (int a = 0; a < NAP; ++a)
+#pragma unroll
+              for (int b = 0; b < NB; ++b) t += V(a, b, c, d) * dfix_s[(A0 + a) * NB + b];
+            if (t != 0.0) fock_add(F, gc[c], gd[d], 4.0 * s * t);
+          }
+      } else {
+        double dab[NAP * NB];
+#pragma unroll
+        for (int a = 0; a < NAP; ++a)
+#pragma unroll
+          for (int b = 0; b < NB; ++b) dab[a * NB + b] = Dm[tri(ga[a], gb[b])];
+#pragma unroll
+        for (int c = 0; c < NCP; ++c)
+#pragma unroll
+          for (int d = 0; d < ND; ++d) {
+            double t = 0.0;
+#pragma unroll
+            for (int a = 0; a < NAP; ++a)
+#pragma unroll
+              for (int b = 0; b < NB; ++b) t += V(a, b, c, d) * dab[a * NB + b];
+            jf[c * ND + d] = 4.0 * s * t;
+          }
+#pragma unroll
+        for (int a = 0; a < NAP; ++a)
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            double t = 0.0;
+#pragma unroll
+            for (int c = 0; c < NCP; ++c)
+#pragma unroll
+              for (int d = 0; d < ND; ++d) t += V(a, b, c, d) * dfix_s[(C0 + c) * ND + d];
+            if (t != 0.0) fock_add(F, ga[a], gb[b], 4.0 * s * t);
+          }
+      }
+      const double sk = s * ba.k_scale;
+      // exchange: K(AC) and K(BD)
+      {
+        double dbd[NB * ND], dac[NAP * NCP];
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+          for (int d = 0; d < ND; ++d) dbd[b * ND + d] = Dk[tri(gb[b], gd[d])];
+#pragma unroll
+        for (int a = 0; a < NAP; ++a)
+#pragma unroll
+          for (int c = 0; c < NCP; ++c) dac[a * NCP + c] = Dk[tri(ga[a], gc[c])];
+#pragma unroll
+        for (int a = 0; a < NAP; ++a)
+#pragma unroll
+          for (int c = 0; c < NCP; ++c) {
+            double t = 0.0;
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+              for (int d = 0; d < ND; ++d) t += V(a, b, c, d) * dbd[b * ND + d];
+            if (t != 0.0)
+              lds_add(TR ? &rows[(C0 + c) * ncols + cm1[a]] : &rows[(A0 + a) * ncols + cm1[c]], -sk * t);
+          }
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+          for (int d = 0; d < ND; ++d) {
+            double t = 0.0;
+#pragma unroll
+            for (int a = 0; a < NAP; ++a)
+#pragma unroll
+              for (int c = 0; c < NCP; ++c) t += V(a, b, c, d) * dac[a * NCP + c];
+            if (t != 0.0) lds_add(TR ? &rows[(NC + d) * ncols + cm2[b]] : &rows[(NA + b) * ncols + cm2[d]], -sk * t);
+          }
+      }
+      // exchange: K(AD) and K(BC)
+      {
+        double dbc[NB * NCP], dad[NAP * ND];
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+          for (int c = 0; c < NCP; ++c) dbc[b * NCP + c] = Dk[tri(gb[b], gc[c])];
+#pragma unroll
+        for (int a = 0; a < NAP; ++a)
+#pragma unroll
+          for (int d = 0; d < ND; ++d) dad[a * ND + d] = Dk[tri(ga[a], gd[d])];
+#pragma unroll
+        for (int a = 0; a < NAP; ++a)
+#pragma unroll
+          for (int d = 0; d < ND; ++d) {
+            double t = 0.0;
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+              for (int c = 0; c < NCP; ++c) t += V(a, b, c, d) * dbc[b * NCP + c];
+            if (t != 0.0) lds_add(TR ? &rows[(NC + d) * ncols + cm1[a]] : &rows[(A0 + a) * ncols + cm2[d]], -sk * t);
+          }
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+          for (int c = 0; c < NCP; ++c) {
+            double t = 0.0;
+#pragma unroll
+            for (int a = 0; a < NAP; ++a)
+#pragma unroll
+              for (int d = 0; d < ND; ++d) t += V(a, b, c, d) * dad[a * ND + d];
+            if (t != 0.0) lds_add(TR ? &rows[(C0 + c) * ncols + cm2[b]] : &rows[(NA + b) * ncols + cm1[c]], -sk * t);
+          }
+      }
+    }
+    // this part's share of the fixed pair's Coulomb block: one wave sum per
+    // element (every lane takes part; lanes past the tile hold zeros)
+#pragma unroll
+    for (int k = 0; k < NJ; ++k) {
+      const double x = wave_sum_dpp(jf[k]);
+      const int fk = TR ? (C0 + k / ND) * ND + k % ND : (A0 + k / NB) * NB + k % NB;
+      if (lane == 63 && x != 0.0) lds_add(&jfix_s[fk], x);
+    }
+  }
+}
+
+// the parts of wave slice wpart (S / W consecutive parts), one after the other
+template <int LA, int LB, int LC, int LD, bool TR, int PART>
+__device__ __forceinline__ void split_dispatch(int wpart, const ClassDev &cd, const BuildArgs &ba,
+                                               const PairRec &fix, int32_t ifix, int64_t q0, int64_t q1, int seg0,
+                                               int bat, double *jfix_s, const double *dfix_s, double *rows,
+                                               int ncols, int off2, const double *tab) {
+  using P = SplitShape<LA, LB, LC, LD>;
+  if constexpr (PART < P::S) {
+    if (PART / (P::S / P::W) == wpart)
+      split_part<LA, LB, LC, LD, TR, PART>(cd, ba, fix, ifix, q0, q1, seg0, bat, jfix_s, dfix_s, rows, ncols, off2,
+                                           tab);
+    split_dispatch<LA, LB, LC, LD, TR, PART + 1>(wpart, cd, ba, fix, ifix, q0, q1, seg0, bat, jfix_s, dfix_s, rows,
+                                                 ncols, off2, tab);
+  }
+}
+
+// LDS of the wave-split kernel: jfix and dfix (the fixed pair's Coulomb sums
+// and density block), the exchange rows, the Rys table below the switch
+template <int LA, int LB, int LC, int LD, bool TR>
+int64_t split_lds(const int32_t *ncol, double xswitch) {
+  using R = RowShape<LA, LB, LC, LD, TR>;
+  return (int64_t)sizeof(double) * (2 * R::NFIX + R::NROW * row_columns<LA, LB, LC, LD, TR>(ncol) +
+                                    rys_table_used(SplitShape<LA, LB, LC, LD>::NR, xswitch));
+}
+
+// Wave-split row-tiled kernel: one workgroup owns one row tile (a run of
+// quartets sharing the fixed pair); wave w takes the parts of wave slice
+// w % W of the quartets of run w / W of each step.
+template <int LA, int LB, int LC, int LD, bool TR>
+__global__ __launch_bounds__((SplitShape<LA, LB, LC, LD>::NT)) void eri_split_kernel(ClassDev cd, BuildArgs ba) {
+  using P = SplitShape<LA, LB, LC, LD>;
+  using R = RowShape<LA, LB, LC, LD, TR>;
+  constexpr int N1 = R::N1, N2 = R::N2, NFIX = R::NFIX, NROW = R::NROW;
+  extern __shared__ double lds[];
+  double *jfix_s = lds, *dfix_s = lds + NFIX, *rows = lds + 2 * NFIX;
+  const int ncols = (int)row_columns<LA, LB, LC, LD, TR>(ba.ncol);
+  const int off2 = R::M1 == R::M2 ? 0 : ba.ncol[R::M1];
+  double *tab = rows + NROW * ncols;
+  {
+    constexpr int IN = 2 * P::NR * (RYS_DEG + 1), LS = rys_lds_stride(P::NR);
+    const int ntab = (int)rys_table_intervals(P::NR, cd.xswitch) * IN;
+    const double *src = rys_table + rys_offset[P::NR];
+    for (int i = threadIdx.x; i < ntab; i += blockDim.x) tab[(i / IN) * LS + i % IN] = src[i];
+  }
+  const int tile = cd.t_begin + (int)blockIdx.x;
+  const int64_t q0 = cd.tile_q[tile] > cd.q_begin ? cd.tile_q[tile] : cd.q_begin;
+  const int64_t q1 = cd.tile_q[tile + 1] < cd.q_end ? cd.tile_q[tile + 1] : cd.q_end;
+  const int seg0 = cd.tile_seg[tile];
+  const int32_t ifix = cd.seg_bra[seg0];
+  const PairRec fix = ba.pairs[ifix];
+  for (int i = threadIdx.x; i < NFIX; i += blockDim.x) {
+    jfix_s[i] = 0.0;
+    dfix_s[i] = ba.D[tri(fix.bfA + gpos_of(R::L1, i / N2), fix.bfB + gpos_of(R::L2, i % N2))];
+  }
+  for (int i = threadIdx.x; i < NROW * ncols; i += blockDim.x) rows[i] = 0.0;
+  __syncthreads();
+
+  // slice and run of this wave (uniform over the wave)
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  split_dispatch<LA, LB, LC, LD, TR, 0>(wave % P::W, cd, ba, fix, ifix, q0, q1, seg0, wave / P::W, jfix_s, dfix_s,
+                                        rows, ncols, off2, tab);
+  __syncthreads();
+
+  double *F = ba.F;
+  for (int i = threadIdx.x; i < NFIX; i += blockDim.x) {
+    const double x = jfix_s[i];
+    if (x != 0.0) fock_add(F, fix.bfA + gpos_of(R::L1, i / N2), fix.bfB + gpos_of(R::L2, i % N2), x);
+  }
+  for (int i = threadIdx.x; i < NROW * ncols; i += blockDim.x) {
+    const double x = rows[i];
+    if (x == 0.0) continue;
+    const int r = i / ncols, k = i - r * ncols;
+    const int rbf = r < N1 ? fix.bfA + gpos_of(R::L1, r) : fix.bfB + gpos_of(R::L2, r - N1);
+    const int cbf = (R::M1 == R::M2 || k < off2) ? ba.col_bf[R::M1][k] : ba.col_bf[R::M2][k - off2];
+    fock_add(F, rbf, cbf, x);
+  }
+}
+
 // ================================================ workgroup-per-quartet path
 __device__ __forceinline__ void cart_decode(int L, int m, int &lx, int &ly, int &lz) {
   int i = 0;
@@ -2374,18 +2892,24 @@ int64_t row_acc_bytes(const int32_t *ncol) {
   return (int64_t)sizeof(double) * (R::NFIX + R::NROW * row_columns<LA, LB, LC, LD, TR>(ncol));
 }
 
-// LDS bytes of the row-tiled form of the class in orientation TR (lane
-// classes only), or -1 when the class has none
+// LDS bytes of the row-tiled form of the class in orientation TR (the lane
+// kernel, or the wave-split kernel of a group class), or -1 when the class
+// has none or its LDS exceeds what that kernel may take
 template <int LA, int LB, int LC, int LD, bool TR>
 int64_t row_lds(const int32_t *ncol, double xswitch) {
   using R = RowShape<LA, LB, LC, LD, TR>;
-  if constexpr (!R::exists)
+  if constexpr (!R::exists) {
     return -1;
-  else if constexpr (use_small<LA, LB, LC, LD>())
-    return row_acc_bytes<LA, LB, LC, LD, TR>(ncol) +
-           (int64_t)sizeof(double) * rys_table_used(Small<LA, LB, LC, LD>::NR, xswitch);
-  else
+  } else if constexpr (use_small<LA, LB, LC, LD>()) {
+    const int64_t b = row_acc_bytes<LA, LB, LC, LD, TR>(ncol) +
+                      (int64_t)sizeof(double) * rys_table_used(Small<LA, LB, LC, LD>::NR, xswitch);
+    return b <= kRowLdsBytes ? b : -1;
+  } else if constexpr (SplitShape<LA, LB, LC, LD>::exists) {
+    const int64_t b = split_lds<LA, LB, LC, LD, TR>(ncol, xswitch);
+    return b <= kSplitLdsBytes ? b : -1;
+  } else {
     return -1;
+  }
 }
 
 // group classes with an f shell also get the cooperative form when its
@@ -2399,12 +2923,21 @@ constexpr bool has_coop_alt() {
 template <int LA, int LB, int LC, int LD, bool TR>
 static int launch_row(const ClassDev &cd, const BuildArgs &ba, hipStream_t st) {
   const int64_t lds = row_lds<LA, LB, LC, LD, TR>(ba.ncol, cd.xswitch);
-  if (lds < 0 || lds > kRowLdsBytes || cd.t_end <= cd.t_begin) return hipErrorInvalidValue;
+  if (lds < 0 || cd.t_end <= cd.t_begin) return hipErrorInvalidValue;
   const dim3 grid((unsigned)(cd.t_end - cd.t_begin));
   if constexpr (!RowShape<LA, LB, LC, LD, TR>::exists) {
     return hipErrorInvalidValue;
   } else if constexpr (use_small<LA, LB, LC, LD>()) {
     hipLaunchKernelGGL((eri_row_kernel<LA, LB, LC, LD, TR>), grid, dim3(256), (unsigned)lds, st, cd, ba);
+  } else if constexpr (SplitShape<LA, LB, LC, LD>::exists) {
+    // more than 64 KB of dynamic LDS needs the kernel's limit raised (once
+    // per process; the call is not a stream operation)
+    static const hipError_t attr = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&eri_split_kernel<LA, LB, LC, LD, TR>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSplitLdsBytes);
+    if (attr != hipSuccess) return attr;
+    hipLaunchKernelGGL((eri_split_kernel<LA, LB, LC, LD, TR>), grid, dim3(SplitShape<LA, LB, LC, LD>::NT),
+                       (unsigned)lds, st, cd, ba);
   } else {
     return hipErrorInvalidValue;
   }
@@ -2424,7 +2957,13 @@ static int launch_one(const ClassDev &cd, const BuildArgs &ba, hipStream_t st, i
                          st, cd, ba);
     }
   } else if constexpr (Grp<LA, LB, LC, LD>::fits) {
-    if (cd.row_mode) return hipErrorInvalidValue;
+    if (cd.row_mode) {
+      if constexpr (SplitShape<LA, LB, LC, LD>::exists)
+        return cd.transposed ? launch_row<LA, LB, LC, LD, true>(cd, ba, st)
+                             : launch_row<LA, LB, LC, LD, false>(cd, ba, st);
+      else
+        return hipErrorInvalidValue;
+    }
     if (cd.coop) {
       if constexpr (has_coop_alt<LA, LB, LC, LD>()) {
         using S = Coop<LA, LB, LC, LD>;

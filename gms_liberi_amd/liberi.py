@@ -43,6 +43,7 @@ OPT_MAX_DEVICE_BYTES = 11
 OPT_RANK_SPLIT = 12
 OPT_FUSED = 13
 OPT_TILE_LEN = 14
+OPT_ROW_ORIENT = 15
 
 _STATUS = {1: "EARG", 2: "ESTATE", 3: "ESIZE", 4: "EDEVICE", 5: "ENOMEM"}
 

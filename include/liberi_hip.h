@@ -167,6 +167,11 @@ const char *liberi_c_last_error(void);
                                         per workgroup: 0 (default) timed per class at the first build
                                         (LIBERI_OPT_AUTOTUNE; 4096 where untimed), or 1024, 4096, 16384
                                         for every class.  Read at each build. */
+#define LIBERI_OPT_ROW_ORIENT 15     /* orientation of the row tiles of a non-diagonal class: 0 (default)
+                                        rows over the pair whose Coulomb block is larger, among the
+                                        orientations whose accumulators fit; 1 rows over the bra pairs,
+                                        2 over the ket pairs, where that orientation fits (else as 0).
+                                        Read at setup. */
 #define LIBERI_OPT_MAX_DEVICE_BYTES 11 /* HBM budget of the handle in bytes (0, the default: none).  An
                                           allocation that would take the handle's device data past it
                                           fails with LIBERI_ENOMEM; a setup that fails that way (or any

@@ -392,3 +392,39 @@ def test_row_tile_lengths_give_the_same_fock():
         assert np.max(np.abs(f - out[0])) <= 1e-13 * scale, length
     # F against the reference class by class: test_gpu_parity
     assert abs(0.5 * float(np.dot(case.density, out[0] - case.fock))) <= 1e-9
+
+
+SPLIT_CLASSES = ["int2120", "int2220", "int0122", "int2111", "int2121"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["cfg_ethane.npz", "cfg_cn_chain.npz", "cfg_c60.npz"])
+def test_wave_split_forms_match_group_forms(name):
+    """The wave-split row-tiled form of the d classes -- slices of (e0|f0)
+    (columns, rows, or the cone of one C component) held in registers by the
+    waves of a workgroup, exchange rows in LDS -- in both orientations of the
+    row tiles, equals the group form with direct atomics, class by class."""
+    case = Case.load_npz(GOLD / name)
+    names = liberi.class_names()
+    for orient in (1, 2):
+        h = liberi.Handle()
+        h.set_option(liberi.OPT_ROW_ORIENT, orient)
+        h.setup(case.basis)
+        counts = h.stats()["quartets"]
+        for k in SPLIT_CLASSES:
+            c = names.index(k)
+            if counts[c] == 0:
+                continue
+            h.set_option(liberi.OPT_ONLY_CLASS, c)
+            h.set_option(liberi.OPT_ROW_TILES, 2)
+            split = h.fock_build(case.density)
+            variant = int(h.stats()["class_variant"][c])
+            h.set_option(liberi.OPT_ROW_TILES, 0)
+            direct = h.fock_build(case.density)
+            assert int(h.stats()["class_variant"][c]) == 0
+            diag = k == "int2121"
+            err = float(np.max(np.abs(split - direct)))
+            print(f"{name} orient {orient} {k}: variant {variant}, max|split - group| {err:.2e}", flush=True)
+            assert variant == (1 if diag else orient)
+            assert err <= 1e-12 * max(1.0, float(np.max(np.abs(direct))))
+        h.destroy()
