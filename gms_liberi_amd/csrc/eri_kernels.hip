@@ -932,9 +932,15 @@ __host__ __device__ constexpr int split_buffer(int LA, int LB, int LC, int LD, i
   return m;
 }
 
-// Parts S and waves per run W of the split form of a class (S = 0: none).
-// Chosen so that the slice, the tables and the HRR levels of a part fit the
-// registers of two waves per SIMD (8-wave workgroups) or three (12 waves).
+// Parts S and waves per run W of the split form of a class (S = 0: none),
+// and for the ket cone the directions of D per part (3: all; 1: one, the
+// part is a (c, direction) pair).  Chosen so that the slice, the tables and
+// the HRR levels of a part fit the registers of two waves per SIMD (8-wave
+// workgroups) or three (12 waves).
+template <int LA, int LB, int LC, int LD>
+constexpr int split_cone_dirs() {
+  return LA == 2 && LB == 2 ? 1 : 3;  // (dd|dp), (dd|pp): 31 rows e, so two columns per part
+}
 template <int LA, int LB, int LC, int LD>
 constexpr int split_parts() {
   if constexpr (LA == 2 && LB == 1 && LC == 2 && LD == 0) return 2;  // (dp|ds): 3 of the 6 d columns
@@ -942,6 +948,8 @@ constexpr int split_parts() {
   if constexpr (LA == 1 && LB == 0 && LC == 2 && LD == 2) return 3;  // (ps|dd): one p row
   if constexpr (LA == 2 && LB == 1 && LC == 1 && LD == 1) return 3;  // (dp|pp): cone of one p of C
   if constexpr (LA == 2 && LB == 1 && LC == 2 && LD == 1) return 6;  // (dp|dp): cone of one d of C
+  if constexpr (LA == 2 && LB == 2 && LC == 2 && LD == 1) return 18;  // (dd|dp): one d of C, one direction
+  if constexpr (LA == 2 && LB == 2 && LC == 1 && LD == 1) return 9;   // (dd|pp): one p of C, one direction
   return 0;
 }
 template <int LA, int LB, int LC, int LD>
@@ -949,6 +957,8 @@ constexpr int split_waves() {
   constexpr int S = split_parts<LA, LB, LC, LD>();
   if constexpr (LA == 2 && LB == 1 && LC == 1 && LD == 1) return 1;  // (dp|pp): the 3 parts in turn
   if constexpr (LA == 2 && LB == 1 && LC == 2 && LD == 1) return 2;  // (dp|dp): 3 parts per wave
+  if constexpr (LA == 2 && LB == 2 && LC == 2 && LD == 1) return 2;  // (dd|dp): 9 parts per wave
+  if constexpr (LA == 2 && LB == 2 && LC == 1 && LD == 1) return 1;  // (dd|pp): 9 parts per wave
   return S;
 }
 
@@ -962,49 +972,56 @@ struct SplitShape {
                               : LB == 0 ? SPLIT_ROWS
                               : LD == 1 ? SPLIT_CONE
                                         : SPLIT_NONE;
+  // ket cone: directions of D per part, parts per component c
+  static constexpr int NDI = SIDE == SPLIT_CONE ? split_cone_dirs<LA, LB, LC, LD>() : ncart(LD);
+  static constexpr int PPC = 3 / (NDI > 0 ? NDI : 1);
   static constexpr bool exists = SIDE != SPLIT_NONE && W > 0 && S % W == 0 &&
                                  (SIDE == SPLIT_COLS   ? NF % S == 0
                                   : SIDE == SPLIT_ROWS ? NE % S == 0
-                                                       : S == ncart(LC));
+                                                       : (NDI == 1 || NDI == 3) && S == ncart(LC) * PPC);
   // rows e and columns f of (e0|f0) one part accumulates
   static constexpr int NEP = SIDE == SPLIT_ROWS ? NE / S : NE;
-  static constexpr int NFP = SIDE == SPLIT_COLS ? NF / S : (SIDE == SPLIT_CONE ? 4 : NF);
-  // columns of the bra HRR (the part's columns, or its three (c, d) after the ket step)
-  static constexpr int NFB = SIDE == SPLIT_CONE ? 3 : NFP;
+  static constexpr int NFP = SIDE == SPLIT_COLS ? NF / S : (SIDE == SPLIT_CONE ? 1 + NDI : NF);
   // waves per workgroup: W per run of 64 quartets, NBAT runs; 8 waves when W
   // divides it, else 12 (two or three waves per SIMD)
   static constexpr int NBAT = !exists ? 1 : (8 % W == 0 ? 8 / W : 12 / W);
   static constexpr int NT = 64 * W * NBAT;
   static constexpr int NR = (LA + LB + LC + LD) / 2 + 1;
   // slice of the block after the HRR: rows a (all but in the rows split),
-  // columns c (one per part in the cone, all in the rows split)
+  // columns c (one per part in the cone, all in the rows split), columns d
+  // (the part's directions in the cone, else all)
   static constexpr int NAP = SIDE == SPLIT_ROWS ? NEP : ncart(LA);
   static constexpr int NCP = SIDE == SPLIT_COLS ? NFP : (SIDE == SPLIT_CONE ? 1 : ncart(LC));
-  static constexpr int NB = ncart(LB), ND = ncart(LD);
-  static constexpr int NV = NAP * NB * NCP * ND;
-  static constexpr int NBUF = SIDE == SPLIT_CONE ? split_buffer(LA, LB, LC, 0, NEP, 3) : split_buffer(LA, LB, LC, LD, NEP, NFP);
-  // first component of the slice in shell A (rows split) or C (columns, cone)
+  static constexpr int NB = ncart(LB), ND = ncart(LD), NDP = NDI;
+  static constexpr int NV = NAP * NB * NCP * NDP;
+  static constexpr int NBUF =
+      SIDE == SPLIT_CONE ? split_buffer(LA, LB, LC, 0, NEP, NDP) : split_buffer(LA, LB, LC, LD, NEP, NFP);
+  // first component of the slice in shell A (rows split), C (columns, cone), D (cone)
   template <int PART>
   static constexpr int a0() { return SIDE == SPLIT_ROWS ? PART * NEP : 0; }
   template <int PART>
-  static constexpr int c0() { return SIDE == SPLIT_COLS ? PART * NFP : (SIDE == SPLIT_CONE ? PART : 0); }
+  static constexpr int c0() { return SIDE == SPLIT_COLS ? PART * NFP : (SIDE == SPLIT_CONE ? PART / PPC : 0); }
+  template <int PART>
+  static constexpr int d0() { return SIDE == SPLIT_CONE ? (PART % PPC) * NDI : 0; }
 };
 
-// column j of part PART: its index in range(LC, LC + LD)
+// column j of part PART: its index in range(LC, LC + LD) (in the cone: c,
+// then c + 1i for the part's directions i)
 template <int LA, int LB, int LC, int LD, int PART>
 constexpr int split_col(int j) {
   using P = SplitShape<LA, LB, LC, LD>;
   if (P::SIDE == SPLIT_COLS) return PART * P::NFP + j;
   if (P::SIDE != SPLIT_CONE) return j;
   constexpr Range<LC, LC> rc{};
-  if (j == 0) return PART;
-  const int i = j - 1;
-  return ncart(LC) + cidx(rc.lx[PART] + (i == 0), rc.ly[PART] + (i == 1), rc.lz[PART] + (i == 2));
+  constexpr int c = P::template c0<PART>();
+  if (j == 0) return c;
+  const int i = P::template d0<PART>() + j - 1;
+  return ncart(LC) + cidx(rc.lx[c] + (i == 0), rc.ly[c] + (i == 1), rc.lz[c] + (i == 2));
 }
 
 // One contracted quartet, slice PART: the part's rows and columns of (e0|f0)
 // over the primitive loop, then the HRR of the slice, cartesian scales and
-// the value cutoff.  Result v[((a*NB + b)*NCP + c)*ND + d] with a, c
+// the value cutoff.  Result v[((a*NB + b)*NCP + c)*NDP + d] with a, c, d
 // relative to the slice's first component.
 template <int LA, int LB, int LC, int LD, int PART, bool KET_OUTER>
 __device__ __forceinline__ void split_integrals(const PairRec &bra, const PairRec &ket, const PrimRec *prims,
@@ -1069,14 +1086,16 @@ __device__ __forceinline__ void split_integrals(const PairRec &bra, const PairRe
   constexpr bool in_b1 =
       P::SIDE == SPLIT_CONE ? LB % 2 == 0 : ((LB % 2) != 0) != ((LD % 2) != 0);
   if constexpr (P::SIDE == SPLIT_CONE) {
-    // ket step into b1 as [e][d], then the bra levels on its three columns
+    // ket step into b1 as [e][d], then the bra levels on those columns
+    constexpr int NDP = P::NDP, D0 = P::template d0<PART>();
 #pragma unroll
-    for (int e = 0; e < NEP; ++e) {
-      b1[e * 3 + 0] = acc[e * 4 + 1] + ket.ABx * acc[e * 4];
-      b1[e * 3 + 1] = acc[e * 4 + 2] + ket.ABy * acc[e * 4];
-      b1[e * 3 + 2] = acc[e * 4 + 3] + ket.ABz * acc[e * 4];
-    }
-    hrr_bra<LA, LB, 3>(b1, acc, bra.ABx, bra.ABy, bra.ABz);
+    for (int e = 0; e < NEP; ++e)
+#pragma unroll
+      for (int j = 0; j < NDP; ++j) {
+        const double cd = D0 + j == 0 ? ket.ABx : (D0 + j == 1 ? ket.ABy : ket.ABz);
+        b1[e * NDP + j] = acc[e * NFP + 1 + j] + cd * acc[e * NFP];
+      }
+    hrr_bra<LA, LB, NDP>(b1, acc, bra.ABx, bra.ABy, bra.ABz);
   } else {
     if constexpr (LB > 0) hrr_bra<LA, LB, NFP>(acc, b1, bra.ABx, bra.ABy, bra.ABz);
     if constexpr (LD > 0) {
@@ -1087,8 +1106,8 @@ __device__ __forceinline__ void split_integrals(const PairRec &bra, const PairRe
     }
   }
 
-  constexpr int A0 = P::template a0<PART>(), C0 = P::template c0<PART>();
-  constexpr int NAP = P::NAP, NB = P::NB, NCP = P::NCP, ND = P::ND;
+  constexpr int A0 = P::template a0<PART>(), C0 = P::template c0<PART>(), D0 = P::template d0<PART>();
+  constexpr int NAP = P::NAP, NB = P::NB, NCP = P::NCP, NDP = P::NDP;
   constexpr Range<LA, LA> ra{};
   constexpr Range<LB, LB> rb{};
   constexpr Range<LC, LC> rc{};
@@ -1100,11 +1119,11 @@ __device__ __forceinline__ void split_integrals(const PairRec &bra, const PairRe
 #pragma unroll
       for (int c = 0; c < NCP; ++c)
 #pragma unroll
-        for (int d = 0; d < ND; ++d) {
-          const int aa = A0 + a, cc = C0 + c;
+        for (int d = 0; d < NDP; ++d) {
+          const int aa = A0 + a, cc = C0 + c, dd = D0 + d;
           const double sc = cart_scale(ra.lx[aa], ra.ly[aa], ra.lz[aa]) * cart_scale(rb.lx[b], rb.ly[b], rb.lz[b]) *
-                            cart_scale(rc.lx[cc], rc.ly[cc], rc.lz[cc]) * cart_scale(rd.lx[d], rd.ly[d], rd.lz[d]);
-          const int idx = ((a * NB + b) * NCP + c) * ND + d;
+                            cart_scale(rc.lx[cc], rc.ly[cc], rc.lz[cc]) * cart_scale(rd.lx[dd], rd.ly[dd], rd.lz[dd]);
+          const int idx = ((a * NB + b) * NCP + c) * NDP + d;
           const double x = (in_b1 ? b1[idx] : acc[idx]) * sc;
           v[idx] = fabs(x) < cut ? 0.0 : x;
         }
@@ -1142,19 +1161,19 @@ __device__ __forceinline__ void split_part(const ClassDev &cd, const BuildArgs &
                                            const double *tab) {
   using P = SplitShape<LA, LB, LC, LD>;
   using R = RowShape<LA, LB, LC, LD, TR>;
-  constexpr int NA = ncart(LA), NB = P::NB, NC = ncart(LC), ND = P::ND;
+  constexpr int NA = ncart(LA), NB = P::NB, NC = ncart(LC), ND = P::ND, NDP = P::NDP;
   constexpr int NAP = P::NAP, NCP = P::NCP;
-  constexpr int A0 = P::template a0<PART>(), C0 = P::template c0<PART>();
+  constexpr int A0 = P::template a0<PART>(), C0 = P::template c0<PART>(), D0 = P::template d0<PART>();
   constexpr Range<LA, LA> ra{};
   constexpr Range<LB, LB> rb{};
   constexpr Range<LC, LC> rc{};
   constexpr Range<LD, LD> rd{};
   // the moving pair's shells: M1 (its slice when it is the split one), M2
-  constexpr int NM1 = TR ? NAP : NCP, M10 = TR ? A0 : C0, NM2 = TR ? NB : ND;
+  constexpr int NM1 = TR ? NAP : NCP, M10 = TR ? A0 : C0, NM2 = TR ? NB : NDP, M20 = TR ? 0 : D0;
   constexpr Range<R::M1, R::M1> rm1{};
   constexpr Range<R::M2, R::M2> rm2{};
   // this part's share of the fixed pair's Coulomb block
-  constexpr int NJ = TR ? NCP * ND : NAP * NB;
+  constexpr int NJ = TR ? NCP * NDP : NAP * NB;
   const double *Dm = ba.D, *Dk = ba.DK;
   double *F = ba.F;
   const double sfix = fix.sameAB ? 0.5 : 1.0;
@@ -1177,7 +1196,7 @@ __device__ __forceinline__ void split_part(const ClassDev &cd, const BuildArgs &
       double s = sfix;
       if (mov.sameAB) s *= 0.5;
       if (cd.diag && ifix == imov) s *= 0.5;
-      int ga[NAP], gb[NB], gc[NCP], gd[ND];
+      int ga[NAP], gb[NB], gc[NCP], gd[NDP];
 #pragma unroll
       for (int a = 0; a < NAP; ++a) ga[a] = bra.bfA + gamess_pos(ra.lx[A0 + a], ra.ly[A0 + a], ra.lz[A0 + a]);
 #pragma unroll
@@ -1185,21 +1204,21 @@ __device__ __forceinline__ void split_part(const ClassDev &cd, const BuildArgs &
 #pragma unroll
       for (int c = 0; c < NCP; ++c) gc[c] = ket.bfA + gamess_pos(rc.lx[C0 + c], rc.ly[C0 + c], rc.lz[C0 + c]);
 #pragma unroll
-      for (int d = 0; d < ND; ++d) gd[d] = ket.bfB + gamess_pos(rd.lx[d], rd.ly[d], rd.lz[d]);
+      for (int d = 0; d < NDP; ++d) gd[d] = ket.bfB + gamess_pos(rd.lx[D0 + d], rd.ly[D0 + d], rd.lz[D0 + d]);
       int cm1[NM1], cm2[NM2];
 #pragma unroll
       for (int i = 0; i < NM1; ++i) cm1[i] = mov.cA + gamess_pos(rm1.lx[M10 + i], rm1.ly[M10 + i], rm1.lz[M10 + i]);
 #pragma unroll
-      for (int i = 0; i < NM2; ++i) cm2[i] = off2 + mov.cB + gamess_pos(rm2.lx[i], rm2.ly[i], rm2.lz[i]);
-      auto V = [&](int a, int b, int c, int d) -> double { return v[((a * NB + b) * NCP + c) * ND + d]; };
+      for (int i = 0; i < NM2; ++i) cm2[i] = off2 + mov.cB + gamess_pos(rm2.lx[M20 + i], rm2.ly[M20 + i], rm2.lz[M20 + i]);
+      auto V = [&](int a, int b, int c, int d) -> double { return v[((a * NB + b) * NCP + c) * NDP + d]; };
 
       // Coulomb: the fixed pair's block summed over the wave, the moving pair's to F
       if constexpr (!TR) {
-        double dcd[NCP * ND];
+        double dcd[NCP * NDP];
 #pragma unroll
         for (int c = 0; c < NCP; ++c)
 #pragma unroll
-          for (int d = 0; d < ND; ++d) dcd[c * ND + d] = Dm[tri(gc[c], gd[d])];
+          for (int d = 0; d < NDP; ++d) dcd[c * NDP + d] = Dm[tri(gc[c], gd[d])];
 #pragma unroll
         for (int a = 0; a < NAP; ++a)
 #pragma unroll
@@ -1208,13 +1227,13 @@ __device__ __forceinline__ void split_part(const ClassDev &cd, const BuildArgs &
 #pragma unroll
             for (int c = 0; c < NCP; ++c)
 #pragma unroll
-              for (int d = 0; d < ND; ++d) t += V(a, b, c, d) * dcd[c * ND + d];
+              for (int d = 0; d < NDP; ++d) t += V(a, b, c, d) * dcd[c * NDP + d];
             jf[a * NB + b] = 4.0 * s * t;
           }
 #pragma unroll
         for (int c = 0; c < NCP; ++c)
 #pragma unroll
-          for (int d = 0; d < ND; ++d) {
+          for (int d = 0; d < NDP; ++d) {
             double t = 0.0;
 #pragma unroll
             for (int a = 0; a < NAP; ++a)
@@ -1231,13 +1250,13 @@ __device__ __forceinline__ void split_part(const ClassDev &cd, const BuildArgs &
 #pragma unroll
         for (int c = 0; c < NCP; ++c)
 #pragma unroll
-          for (int d = 0; d < ND; ++d) {
+          for (int d = 0; d < NDP; ++d) {
             double t = 0.0;
 #pragma unroll
             for (int a = 0; a < NAP; ++a)
 #pragma unroll
               for (int b = 0; b < NB; ++b) t += V(a, b, c, d) * dab[a * NB + b];
-            jf[c * ND + d] = 4.0 * s * t;
+            jf[c * NDP + d] = 4.0 * s * t;
           }
 #pragma unroll
         for (int a = 0; a < NAP; ++a)
@@ -1247,18 +1266,18 @@ __device__ __forceinline__ void split_part(const ClassDev &cd, const BuildArgs &
 #pragma unroll
             for (int c = 0; c < NCP; ++c)
 #pragma unroll
-              for (int d = 0; d < ND; ++d) t += V(a, b, c, d) * dfix_s[(C0 + c) * ND + d];
+              for (int d = 0; d < NDP; ++d) t += V(a, b, c, d) * dfix_s[(C0 + c) * ND + D0 + d];
             if (t != 0.0) fock_add(F, ga[a], gb[b], 4.0 * s * t);
           }
       }
       const double sk = s * ba.k_scale;
       // exchange: K(AC) and K(BD)
       {
-        double dbd[NB * ND], dac[NAP * NCP];
+        double dbd[NB * NDP], dac[NAP * NCP];
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
-          for (int d = 0; d < ND; ++d) dbd[b * ND + d] = Dk[tri(gb[b], gd[d])];
+          for (int d = 0; d < NDP; ++d) dbd[b * NDP + d] = Dk[tri(gb[b], gd[d])];
 #pragma unroll
         for (int a = 0; a < NAP; ++a)
 #pragma unroll
@@ -1271,25 +1290,25 @@ __device__ __forceinline__ void split_part(const ClassDev &cd, const BuildArgs &
 #pragma unroll
             for (int b = 0; b < NB; ++b)
 #pragma unroll
-              for (int d = 0; d < ND; ++d) t += V(a, b, c, d) * dbd[b * ND + d];
+              for (int d = 0; d < NDP; ++d) t += V(a, b, c, d) * dbd[b * NDP + d];
             if (t != 0.0)
               lds_add(TR ? &rows[(C0 + c) * ncols + cm1[a]] : &rows[(A0 + a) * ncols + cm1[c]], -sk * t);
           }
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
-          for (int d = 0; d < ND; ++d) {
+          for (int d = 0; d < NDP; ++d) {
             double t = 0.0;
 #pragma unroll
             for (int a = 0; a < NAP; ++a)
 #pragma unroll
               for (int c = 0; c < NCP; ++c) t += V(a, b, c, d) * dac[a * NCP + c];
-            if (t != 0.0) lds_add(TR ? &rows[(NC + d) * ncols + cm2[b]] : &rows[(NA + b) * ncols + cm2[d]], -sk * t);
+            if (t != 0.0) lds_add(TR ? &rows[(NC + D0 + d) * ncols + cm2[b]] : &rows[(NA + b) * ncols + cm2[d]], -sk * t);
           }
       }
       // exchange: K(AD) and K(BC)
       {
-        double dbc[NB * NCP], dad[NAP * ND];
+        double dbc[NB * NCP], dad[NAP * NDP];
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
@@ -1297,17 +1316,17 @@ __device__ __forceinline__ void split_part(const ClassDev &cd, const BuildArgs &
 #pragma unroll
         for (int a = 0; a < NAP; ++a)
 #pragma unroll
-          for (int d = 0; d < ND; ++d) dad[a * ND + d] = Dk[tri(ga[a], gd[d])];
+          for (int d = 0; d < NDP; ++d) dad[a * NDP + d] = Dk[tri(ga[a], gd[d])];
 #pragma unroll
         for (int a = 0; a < NAP; ++a)
 #pragma unroll
-          for (int d = 0; d < ND; ++d) {
+          for (int d = 0; d < NDP; ++d) {
             double t = 0.0;
 #pragma unroll
             for (int b = 0; b < NB; ++b)
 #pragma unroll
               for (int c = 0; c < NCP; ++c) t += V(a, b, c, d) * dbc[b * NCP + c];
-            if (t != 0.0) lds_add(TR ? &rows[(NC + d) * ncols + cm1[a]] : &rows[(A0 + a) * ncols + cm2[d]], -sk * t);
+            if (t != 0.0) lds_add(TR ? &rows[(NC + D0 + d) * ncols + cm1[a]] : &rows[(A0 + a) * ncols + cm2[d]], -sk * t);
           }
 #pragma unroll
         for (int b = 0; b < NB; ++b)
@@ -1317,7 +1336,7 @@ __device__ __forceinline__ void split_part(const ClassDev &cd, const BuildArgs &
 #pragma unroll
             for (int a = 0; a < NAP; ++a)
 #pragma unroll
-              for (int d = 0; d < ND; ++d) t += V(a, b, c, d) * dad[a * ND + d];
+              for (int d = 0; d < NDP; ++d) t += V(a, b, c, d) * dad[a * NDP + d];
             if (t != 0.0) lds_add(TR ? &rows[(C0 + c) * ncols + cm2[b]] : &rows[(NA + b) * ncols + cm1[c]], -sk * t);
           }
       }
@@ -1327,7 +1346,7 @@ __device__ __forceinline__ void split_part(const ClassDev &cd, const BuildArgs &
 #pragma unroll
     for (int k = 0; k < NJ; ++k) {
       const double x = wave_sum_dpp(jf[k]);
-      const int fk = TR ? (C0 + k / ND) * ND + k % ND : (A0 + k / NB) * NB + k % NB;
+      const int fk = TR ? (C0 + k / NDP) * ND + D0 + k % NDP : (A0 + k / NB) * NB + k % NB;
       if (lane == 63 && x != 0.0) lds_add(&jfix_s[fk], x);
     }
   }

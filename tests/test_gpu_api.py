@@ -394,7 +394,7 @@ def test_row_tile_lengths_give_the_same_fock():
     assert abs(0.5 * float(np.dot(case.density, out[0] - case.fock))) <= 1e-9
 
 
-SPLIT_CLASSES = ["int2120", "int2220", "int0122", "int2111", "int2121"]
+SPLIT_CLASSES = ["int2120", "int2220", "int0122", "int2111", "int2121", "int2221", "int2211"]
 
 
 @pytest.mark.gpu
