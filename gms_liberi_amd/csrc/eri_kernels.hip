@@ -145,7 +145,9 @@ __device__ __forceinline__ void rys_rule(double X, double xswitch, double (&u)[N
         w[r - N] = acc;
     }
   } else {
-    const double ix = 1.0 / X, isx = sqrt(ix);
+    // 1/sqrt(X) and 1/X from the hardware estimate with Newton steps (a few
+    // ulp; the IEEE divide and square root cost twice the instructions)
+    const double isx = nr_rsqrt(X), ix = isx * isx;
 #pragma unroll
     for (int r = 0; r < N; ++r) {
       u[r] = rys_herm_u[N][r] * ix;
@@ -239,6 +241,46 @@ __device__ __forceinline__ void decode_quartet(const ClassDev &cd, int64_t q, in
 
 __device__ __forceinline__ void fock_add(double *F, uint32_t i, uint32_t j, double v) {
   unsafeAtomicAdd(F + tri(i, j), v);
+}
+
+// Read-only views of the pair and primitive records in the constant address
+// space: a load at a wave-uniform address (the fixed pair of a row tile and
+// its primitives) then goes through the scalar data cache into SGPRs,
+// instead of one vector load per lane.  The kernels never write the records.
+typedef __attribute__((address_space(4))) const PrimRec ConstPrimRec;
+typedef __attribute__((address_space(4))) const PairRec ConstPairRec;
+__device__ __forceinline__ ConstPrimRec *const_view(const PrimRec *p) { return (ConstPrimRec *)p; }
+__device__ __forceinline__ ConstPairRec *const_view(const PairRec *p) { return (ConstPairRec *)p; }
+// record i of a constant view, by value
+__device__ __forceinline__ PrimRec load_rec(ConstPrimRec *v, int64_t i) {
+  ConstPrimRec *q = v + i;
+  PrimRec r;
+  r.p = q->p;
+  r.Px = q->Px;
+  r.Py = q->Py;
+  r.Pz = q->Pz;
+  r.PAx = q->PAx;
+  r.PAy = q->PAy;
+  r.PAz = q->PAz;
+  r.K = q->K;
+  return r;
+}
+__device__ __forceinline__ PairRec load_rec(ConstPairRec *v, int64_t i) {
+  ConstPairRec *q = v + i;
+  PairRec r;
+  r.prim_off = q->prim_off;
+  r.nprim = q->nprim;
+  r.bfA = q->bfA;
+  r.bfB = q->bfB;
+  r.ABx = q->ABx;
+  r.ABy = q->ABy;
+  r.ABz = q->ABz;
+  r.sameAB = q->sameAB;
+  r.shA = q->shA;
+  r.cA = q->cA;
+  r.cB = q->cB;
+  r.x = q->x;
+  return r;
 }
 
 // ===================================================== lane-per-quartet path
@@ -356,7 +398,7 @@ __device__ __forceinline__ void count_fp32(const BuildArgs &ba, unsigned n) {
 // arithmetic type of the 2-D integrals, the primitive accumulation and the
 // HRR; the Rys argument, roots and weights are always FP64 (they stand where
 // the reference evaluates its Boys function), and so is the result.
-template <class T, int LA, int LB, int LC, int LD, bool KET_OUTER = false>
+template <class T, int LA, int LB, int LC, int LD, bool KET_OUTER = false, bool FIXED_KET = false>
 __device__ __forceinline__ void small_integrals(const PairRec &bra, const PairRec &ket, const PrimRec *prims,
                                                 double xswitch, double cut,
                                                 double (&v)[Small<LA, LB, LC, LD>::NABCD],
@@ -405,10 +447,12 @@ __device__ __forceinline__ void small_integrals(const PairRec &bra, const PairRe
   };
   if constexpr (KET_OUTER) {
     // the inner loop walks the bra pair's primitives (the pair a row tile
-    // keeps fixed: every lane then reads the same records)
+    // keeps fixed: every lane then reads the same records, through the
+    // scalar cache)
+    ConstPrimRec *cprims = const_view(prims);
     for (int iq = 0; iq < ket.nprim; ++iq) {
       const PrimRec Q = prims[ket.prim_off + iq];
-      for (int ip = 0; ip < bra.nprim; ++ip) prim(prims[bra.prim_off + ip], Q);
+      for (int ip = 0; ip < bra.nprim; ++ip) prim(load_rec(cprims, bra.prim_off + ip), Q);
     }
   } else {
     for (int ip = 0; ip < bra.nprim; ++ip) {
@@ -418,10 +462,17 @@ __device__ __forceinline__ void small_integrals(const PairRec &bra, const PairRe
       // classes the extra registers cost more occupancy than the latency it
       // hides)
       constexpr bool ahead = LA + LB + LC + LD == 0;
-      PrimRec Qn = prims[ket.prim_off];
+      // a row tile's fixed pair is the ket here (uniform: the scalar cache)
+      auto ket_rec = [&](int iq) -> PrimRec {
+        if constexpr (FIXED_KET)
+          return load_rec(const_view(prims), ket.prim_off + iq);
+        else
+          return prims[ket.prim_off + iq];
+      };
+      PrimRec Qn = ket_rec(0);
       for (int iq = 0; iq < ket.nprim; ++iq) {
-        const PrimRec Q = ahead ? Qn : prims[ket.prim_off + iq];
-        if (ahead && iq + 1 < ket.nprim) Qn = prims[ket.prim_off + iq + 1];
+        const PrimRec Q = ahead ? Qn : ket_rec(iq);
+        if (ahead && iq + 1 < ket.nprim) Qn = ket_rec(iq + 1);
         prim(P, Q);
       }
     }
@@ -699,7 +750,7 @@ __global__ __launch_bounds__(256) void eri_row_kernel(ClassDev cd, BuildArgs ba)
   const int64_t q1 = cd.tile_q[tile + 1] < cd.q_end ? cd.tile_q[tile + 1] : cd.q_end;
   const int seg0 = cd.tile_seg[tile];
   const int32_t ifix = cd.seg_bra[seg0];
-  const PairRec fix = ba.pairs[ifix];
+  const PairRec fix = load_rec(const_view(ba.pairs), ifix);
   for (int i = threadIdx.x; i < NFIX + NROW * ncols; i += blockDim.x) lds[i] = 0.0;
 
   const double *Dm = ba.D, *Dk = ba.DK;  // Coulomb and exchange densities
@@ -730,10 +781,10 @@ __global__ __launch_bounds__(256) void eri_row_kernel(ClassDev cd, BuildArgs ba)
     double v[S::NABCD];
     if (mixed_wave(cd, ba, bra, ket)) {
       ++nlow;
-      small_integrals<float, LA, LB, LC, LD, !TR>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, v, tab,
+      small_integrals<float, LA, LB, LC, LD, !TR, TR>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, v, tab,
                                              rys_lds_stride(S::NR));
     } else {
-      small_integrals<double, LA, LB, LC, LD, !TR>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, v, tab,
+      small_integrals<double, LA, LB, LC, LD, !TR, TR>(bra, ket, ba.prims, cd.xswitch, ba.value_cut, v, tab,
                                               rys_lds_stride(S::NR));
     }
 
@@ -916,6 +967,7 @@ __global__ __launch_bounds__(256) void eri_row_kernel(ClassDev cd, BuildArgs ba)
 
 enum SplitSide { SPLIT_NONE = 0, SPLIT_COLS = 1, SPLIT_ROWS = 2, SPLIT_CONE = 3 };
 
+
 // largest HRR intermediate of a slice: NEP rows e and NFP columns f of
 // (e0|f0), bra levels on the NFP columns, ket levels on the slice's rows ab
 __host__ __device__ constexpr int split_buffer(int LA, int LB, int LC, int LD, int NEP, int NFP) {
@@ -1067,15 +1119,17 @@ __device__ __forceinline__ void split_integrals(const PairRec &bra, const PairRe
         }
     }
   };
+  // the fixed pair's primitives (the inner loop) through the scalar cache
+  ConstPrimRec *cprims = const_view(prims);
   if constexpr (KET_OUTER) {
     for (int iq = 0; iq < ket.nprim; ++iq) {
       const PrimRec Q = prims[ket.prim_off + iq];
-      for (int ip = 0; ip < bra.nprim; ++ip) prim(prims[bra.prim_off + ip], Q);
+      for (int ip = 0; ip < bra.nprim; ++ip) prim(load_rec(cprims, bra.prim_off + ip), Q);
     }
   } else {
     for (int ip = 0; ip < bra.nprim; ++ip) {
       const PrimRec Pp = prims[bra.prim_off + ip];
-      for (int iq = 0; iq < ket.nprim; ++iq) prim(Pp, prims[ket.prim_off + iq]);
+      for (int iq = 0; iq < ket.nprim; ++iq) prim(Pp, load_rec(cprims, ket.prim_off + iq));
     }
   }
 
@@ -1401,7 +1455,7 @@ __global__ __launch_bounds__((SplitShape<LA, LB, LC, LD>::NT)) void eri_split_ke
   const int64_t q1 = cd.tile_q[tile + 1] < cd.q_end ? cd.tile_q[tile + 1] : cd.q_end;
   const int seg0 = cd.tile_seg[tile];
   const int32_t ifix = cd.seg_bra[seg0];
-  const PairRec fix = ba.pairs[ifix];
+  const PairRec fix = load_rec(const_view(ba.pairs), ifix);
   for (int i = threadIdx.x; i < NFIX; i += blockDim.x) {
     jfix_s[i] = 0.0;
     dfix_s[i] = ba.D[tri(fix.bfA + gpos_of(R::L1, i / N2), fix.bfB + gpos_of(R::L2, i % N2))];
