@@ -12,7 +12,7 @@ primitive-quartet rates, the roofline of the dominant class kernel (FP64
 VALU), the accuracy of the build against the reference's own Fock matrix,
 the host-array entry point's rate (D upload and F download included), the
 mixed-precision sweep over the FP32 switch tau, the time of each of 8 rank
-shares, the C-N chain (f classes) timed alone, a serving stream, and a CPU
+shares, the C-N chain (f classes), ethane and water workloads timed alone, a serving stream, and a CPU
 baseline: the reference CPU path (flang build, oracle/_ref) timed on this
 host on the whole workload (median of 3 builds), with its thread scaling on a
 12-atom piece of the cage.
@@ -326,7 +326,7 @@ def main():
     ap.add_argument("--taus", default="1e-4,1e-5,1e-6,1e-7",
                     help="FP32 switches of the mixed-precision sweep (comma separated; empty: skip)")
     ap.add_argument("--tau", type=float, default=1e-6, help="FP32 switch of the mixed serving line")
-    ap.add_argument("--extra", default="cn_chain", help="further workloads timed alone (comma separated)")
+    ap.add_argument("--extra", default="cn_chain,ethane,water", help="further workloads timed alone (comma separated)")
     ap.add_argument("--rank-shares", type=int, default=8, help="rank shares of the workload timed one by one (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=12, help="requests of the serving stream (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=1.0, help="arrivals per second of the serving stream")
