@@ -717,8 +717,21 @@ __host__ __device__ inline int64_t row_columns(const int32_t *ncol) {
 // added to F once per tile.  The Coulomb block of the fixed pair is summed in
 // registers over the tile and reduced across the workgroup at the end.  Only
 // the Coulomb block of the moving pair goes to F per quartet.
+// Threads of a row-tiled lane-per-quartet workgroup: one copy of the
+// tile's LDS rows and Rys table serves all of them, so larger workgroups
+// hold more waves per CU where the LDS limits them.  Measured on C60 (256 ->
+// 512 / 1024 threads): (pp|ds) 4.61 -> 4.18 ms, (pp|pp) 2.18 -> 1.97, (ps|ps)
+// 6.09 -> 5.83 (1024); the other lane classes are as fast or faster at 256.
+template <int LA, int LB, int LC, int LD>
+constexpr int row_threads() {
+  if constexpr (LA == 1 && LB == 1 && LC == 2 && LD == 0) return 512;
+  if constexpr (LA == 1 && LB == 1 && LC == 1 && LD == 1) return 512;
+  if constexpr (LA == 1 && LB == 0 && LC == 1 && LD == 0) return 1024;
+  return 256;
+}
+
 template <int LA, int LB, int LC, int LD, bool TR>
-__global__ __launch_bounds__(256) void eri_row_kernel(ClassDev cd, BuildArgs ba) {
+__global__ __launch_bounds__((row_threads<LA, LB, LC, LD>())) void eri_row_kernel(ClassDev cd, BuildArgs ba) {
   using S = Small<LA, LB, LC, LD>;
   using R = RowShape<LA, LB, LC, LD, TR>;
   constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND;
@@ -2973,13 +2986,13 @@ int64_t row_lds(const int32_t *ncol, double xswitch) {
   using R = RowShape<LA, LB, LC, LD, TR>;
   if constexpr (!R::exists) {
     return -1;
+  } else if constexpr (SplitShape<LA, LB, LC, LD>::exists) {
+    const int64_t b = split_lds<LA, LB, LC, LD, TR>(ncol, xswitch);
+    return b <= kSplitLdsBytes ? b : -1;
   } else if constexpr (use_small<LA, LB, LC, LD>()) {
     const int64_t b = row_acc_bytes<LA, LB, LC, LD, TR>(ncol) +
                       (int64_t)sizeof(double) * rys_table_used(Small<LA, LB, LC, LD>::NR, xswitch);
     return b <= kRowLdsBytes ? b : -1;
-  } else if constexpr (SplitShape<LA, LB, LC, LD>::exists) {
-    const int64_t b = split_lds<LA, LB, LC, LD, TR>(ncol, xswitch);
-    return b <= kSplitLdsBytes ? b : -1;
   } else {
     return -1;
   }
@@ -3000,8 +3013,9 @@ static int launch_row(const ClassDev &cd, const BuildArgs &ba, hipStream_t st) {
   const dim3 grid((unsigned)(cd.t_end - cd.t_begin));
   if constexpr (!RowShape<LA, LB, LC, LD, TR>::exists) {
     return hipErrorInvalidValue;
-  } else if constexpr (use_small<LA, LB, LC, LD>()) {
-    hipLaunchKernelGGL((eri_row_kernel<LA, LB, LC, LD, TR>), grid, dim3(256), (unsigned)lds, st, cd, ba);
+  } else if constexpr (use_small<LA, LB, LC, LD>() && !SplitShape<LA, LB, LC, LD>::exists) {
+    hipLaunchKernelGGL((eri_row_kernel<LA, LB, LC, LD, TR>), grid, dim3(row_threads<LA, LB, LC, LD>()), (unsigned)lds,
+                       st, cd, ba);
   } else if constexpr (SplitShape<LA, LB, LC, LD>::exists) {
     // more than 64 KB of dynamic LDS needs the kernel's limit raised (once
     // per process; the call is not a stream operation)
