@@ -2155,6 +2155,12 @@ constexpr int grp_pad(int map, int p, int nq, int g, int nr, int gsz0, int bufp,
   return best;
 }
 
+// classes whose group form runs the HRR in registers (Grp::REG): one
+// column of (e0|f0) and one row (ab| of the ket phase per lane
+__host__ __device__ constexpr bool grp_reg_hrr(int LA, int LB, int LC, int LD) {
+  return LA == 2 && LB == 2 && LC == 2 && LD == 2;
+}
+
 template <int LA, int LB, int LC, int LD>
 struct Grp {
   static constexpr int LAB = LA + LB, LCD = LC + LD;
@@ -2169,15 +2175,26 @@ struct Grp {
   // row stride of the ket-phase arrays [ab][...]: the longest ket-phase row,
   // made odd so that lanes on consecutive rows hit distinct LDS banks
   static constexpr int KS = odd_at_least(ket_row_max(LC, LD));
+  // register form of the HRR (REG): the bra HRR runs one column f of
+  // (e0|f0) per lane and the ket HRR one row ab per lane, both in registers;
+  // LDS holds (e0|f0) as [e][f], then (ab|f0) as [ab][f] (odd row stride TS),
+  // then the digestion's partial sums ([ab][cd] for J(CD), stride JS, and
+  // the four exchange blocks per row ab).  For the classes whose columns and
+  // rows fit a lane ((dd|dd)).
+  static constexpr bool REG = grp_reg_hrr(LA, LB, LC, LD);
+  static constexpr int TS = odd_at_least(NF), JS = odd_at_least(NCD);
+  static constexpr int REGA = NE * NF + NAB * TS, REGB = NAB * JS + NAB * 2 * (NC + ND);
+  static constexpr int CBUF = hrr_max_buffer(LA, LB, 0, 0), RBUF = hrr_max_buffer(0, 0, LC, LD);
   // buffers of the HRR: bra phase [e][b][f] (stride NF), ket phase [ab][KS]
-  static constexpr int BUFP = grp_buffer(LA, LB, LC, LD, KS);
+  // (REG: half of the larger of the two layouts above)
+  static constexpr int BUFP = REG ? ((REGA > REGB ? REGA : REGB) + 1) / 2 : grp_buffer(LA, LB, LC, LD, KS);
   // slot before padding: table buffers (at least two), dead once the
   // primitive loop ends, aliased by the HRR buffers; the density blocks of
   // the digestion go into the HRR buffer the last level left free when they
-  // fit there (DBX = 0), else after the buffers (DBX doubles)
+  // fit there (DBX = 0), else after the buffers (DBX doubles; always in REG)
   static constexpr int GSZ0 = NI * NK;
   static constexpr int UNI0 = 2 * 3 * NR * GSZ0 > 2 * BUFP ? 2 * 3 * NR * GSZ0 : 2 * BUFP;
-  static constexpr int DBX = DB <= BUFP ? 0 : DB;
+  static constexpr int DBX = REG || DB > BUFP ? DB : 0;
   static constexpr int SLOT0 = UNI0 + DBX;
   // lane map of the primitive loop.  MAP 1: a lane per row e of (e0|f0)
   // accumulating all its columns f; MAP 2: a lane per column f accumulating
@@ -2275,6 +2292,7 @@ __device__ __forceinline__ void grp_run(const ClassDev &cd, const BuildArgs &ba,
   constexpr int GSZ = S::GSZ;  // stride of the 2-D tables (padded against bank conflicts)
   constexpr int RT = S::RT, REC = S::REC, KT = S::KT;
   static_assert(RT >= KT && KT * S::GS <= S::UNI, "no room for the tables and rule records");
+  static_assert(!S::REG || (S::NAB <= G && LB > 0 && 2 * S::BUFP <= S::UNI), "register HRR: one row ab per lane");
 
   // lanes past the NQ groups (G need not divide 64) idle: their gl lies past
   // every loop bound and their slots are never valid
@@ -2502,6 +2520,141 @@ __device__ __forceinline__ void grp_run(const ClassDev &cd, const BuildArgs &ba,
     }
     wave_sync();
 
+    if constexpr (S::REG) {
+      constexpr int NAB = S::NAB, NCD = S::NCD, TS = S::TS, JS = S::JS;
+      // the six density blocks, after the buffers
+      double *DAB = Gt + S::UNI, *DCD = DAB + NAB, *DAC = DCD + NCD, *DBD = DAC + S::NAC, *DAD = DBD + S::NBD,
+             *DBC = DAD + S::NAD;
+      if (valid) {
+        const double *Dm = ba.D, *Dk = ba.DK;  // Coulomb and exchange densities
+        for (int i = gl; i < NAB; i += G) DAB[i] = Dm[tri(bra.bfA + gpos_rt(LA, i / NB), bra.bfB + gpos_rt(LB, i % NB))];
+        for (int i = gl; i < NCD; i += G) DCD[i] = Dm[tri(ket.bfA + gpos_rt(LC, i / ND), ket.bfB + gpos_rt(LD, i % ND))];
+        for (int i = gl; i < S::NAC; i += G) DAC[i] = Dk[tri(bra.bfA + gpos_rt(LA, i / NC), ket.bfA + gpos_rt(LC, i % NC))];
+        for (int i = gl; i < S::NBD; i += G) DBD[i] = Dk[tri(bra.bfB + gpos_rt(LB, i / ND), ket.bfB + gpos_rt(LD, i % ND))];
+        for (int i = gl; i < S::NAD; i += G) DAD[i] = Dk[tri(bra.bfA + gpos_rt(LA, i / ND), ket.bfB + gpos_rt(LD, i % ND))];
+        for (int i = gl; i < S::NBC; i += G) DBC[i] = Dk[tri(bra.bfB + gpos_rt(LB, i / NC), ket.bfA + gpos_rt(LC, i % NC))];
+      }
+      // bra HRR: one column f per lane, in registers; (ab|f) to [ab][f]
+      double *bufT = Gt + NE * NF;
+      for (int f = gl; f < NF; f += G) {
+        double c0[S::CBUF], c1[S::CBUF];
+#pragma unroll
+        for (int e = 0; e < NE; ++e) c0[e] = buf0[e * NF + f];
+        hrr_bra<LA, LB, 1>(c0, c1, bra.ABx, bra.ABy, bra.ABz);
+#pragma unroll
+        for (int ab = 0; ab < NAB; ++ab) bufT[ab * TS + f] = (LB % 2) ? c1[ab] : c0[ab];
+      }
+      wave_sync();
+      // ket HRR: one row ab per lane, in registers, then the cartesian
+      // scales and the value cutoff
+      const bool row = valid && gl < NAB;
+      double v[NCD];
+      if (row) {
+        double r0[S::RBUF], r1[S::RBUF];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) r0[f] = bufT[gl * TS + f];
+        hrr_ket<LC, LD, 1>(r0, r1, ket.ABx, ket.ABy, ket.ABz);
+        constexpr Range<LC, LC> rc{};
+        constexpr Range<LD, LD> rd{};
+        const double sab = scale_rt(LA, gl / NB) * scale_rt(LB, gl % NB);
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+          for (int d = 0; d < ND; ++d) {
+            const double x = ((LD % 2) ? r1[c * ND + d] : r0[c * ND + d]) * sab *
+                             (cart_scale(rc.lx[c], rc.ly[c], rc.lz[c]) * cart_scale(rd.lx[d], rd.ly[d], rd.lz[d]));
+            v[c * ND + d] = fabs(x) < ba.value_cut ? 0.0 : x;
+          }
+      }
+      wave_sync();  // [ab][f] read: its space takes the partial sums
+      // digestion: each lane's row ab contributes to the six blocks; the
+      // sums over the other rows go through LDS
+      double s = 1.0;
+      if (bra.sameAB) s *= 0.5;
+      if (ket.sameAB) s *= 0.5;
+      if (cd.diag && ib == ik) s *= 0.5;
+      double *JC = Gt, *KAC = JC + NAB * JS, *KBD = KAC + NAB * NC, *KAD = KBD + NAB * ND, *KBC = KAD + NAB * ND;
+      if (valid && jab_pair != ib) {
+        flush_jab();
+        jab_pair = ib;
+        jab_a = bra.bfA;
+        jab_b = bra.bfB;
+      }
+      if (row) {
+        const int a = gl / NB, b = gl - a * NB;
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < NCD; ++k) t += v[k] * DCD[k];
+        jab[0] += 4.0 * s * t;  // J(AB): the row's own element, in registers over the bra pair's run
+        const double dab = DAB[gl];
+#pragma unroll
+        for (int k = 0; k < NCD; ++k) JC[gl * JS + k] = v[k] * dab;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          double kac = 0.0, kbc = 0.0;
+#pragma unroll
+          for (int d = 0; d < ND; ++d) {
+            kac += v[c * ND + d] * DBD[b * ND + d];
+            kbc += v[c * ND + d] * DAD[a * ND + d];
+          }
+          KAC[(a * NB + b) * NC + c] = kac;
+          KBC[(b * NA + a) * NC + c] = kbc;
+        }
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+          double kbd = 0.0, kad = 0.0;
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            kbd += v[c * ND + d] * DAC[a * NC + c];
+            kad += v[c * ND + d] * DBC[b * NC + c];
+          }
+          KBD[(b * NA + a) * ND + d] = kbd;
+          KAD[(a * NB + b) * ND + d] = kad;
+        }
+      }
+      wave_sync();
+      if (valid) {
+        double *F = ba.F;
+        const double sk = s * ba.k_scale;
+        auto put = [&](int64_t fi, int64_t fj, double x) {
+          if (x != 0.0) fock_add(F, fi, fj, x);
+        };
+        for (int i = gl; i < NCD; i += G) {
+          double t = 0.0;
+#pragma unroll
+          for (int ab = 0; ab < NAB; ++ab) t += JC[ab * JS + i];
+          put(ket.bfA + gpos_rt(LC, i / ND), ket.bfB + gpos_rt(LD, i % ND), 4.0 * s * t);
+        }
+        for (int i = gl; i < NA * NC; i += G) {
+          const int a = i / NC, c = i - a * NC;
+          double t = 0.0;
+#pragma unroll
+          for (int b = 0; b < NB; ++b) t += KAC[(a * NB + b) * NC + c];
+          put(bra.bfA + gpos_rt(LA, a), ket.bfA + gpos_rt(LC, c), -sk * t);
+        }
+        for (int i = gl; i < NB * ND; i += G) {
+          const int b = i / ND, d = i - b * ND;
+          double t = 0.0;
+#pragma unroll
+          for (int a = 0; a < NA; ++a) t += KBD[(b * NA + a) * ND + d];
+          put(bra.bfB + gpos_rt(LB, b), ket.bfB + gpos_rt(LD, d), -sk * t);
+        }
+        for (int i = gl; i < NA * ND; i += G) {
+          const int a = i / ND, d = i - a * ND;
+          double t = 0.0;
+#pragma unroll
+          for (int b = 0; b < NB; ++b) t += KAD[(a * NB + b) * ND + d];
+          put(bra.bfA + gpos_rt(LA, a), ket.bfB + gpos_rt(LD, d), -sk * t);
+        }
+        for (int i = gl; i < NB * NC; i += G) {
+          const int b = i / NC, c = i - b * NC;
+          double t = 0.0;
+#pragma unroll
+          for (int a = 0; a < NA; ++a) t += KBC[(b * NA + a) * NC + c];
+          put(bra.bfB + gpos_rt(LB, b), ket.bfA + gpos_rt(LC, c), -sk * t);
+        }
+      }
+    } else {
     // bra HRR, lanes over the columns f; the last level writes rows [ab] of
     // stride KS for the ket HRR, whose lanes run over the rows ab.  Component
     // loops are compile-time, so every LDS address is a lane base plus a
@@ -2697,6 +2850,7 @@ __device__ __forceinline__ void grp_run(const ClassDev &cd, const BuildArgs &ba,
         }
       }
     }
+    }
     wave_sync();  // slots are reused by the next round
   };
 
@@ -2720,7 +2874,10 @@ __device__ __forceinline__ void grp_run(const ClassDev &cd, const BuildArgs &ba,
 }
 
 template <int LA, int LB, int LC, int LD>
-__attribute__((amdgpu_waves_per_eu(LA <= 2 && LB <= 2 && LC <= 2 && LD <= 2 && LA + LB + LC + LD <= 7 ? 3 : 1, 8)))
+__attribute__((amdgpu_waves_per_eu(grp_reg_hrr(LA, LB, LC, LD)                                              ? 2
+                                   : LA <= 2 && LB <= 2 && LC <= 2 && LD <= 2 && LA + LB + LC + LD <= 7 ? 3
+                                                                                                        : 1,
+                                   8)))
 __global__ __launch_bounds__(64) void eri_group_kernel(ClassDev cd, BuildArgs ba) {
   using S = Grp<LA, LB, LC, LD>;
   constexpr int NQ = S::NQ;
