@@ -257,6 +257,21 @@ const char *liberi_c_last_error(void) { return g_last_error.c_str(); }
 
 const char *liberi_c_class_name(int32_t c) { return (c >= 0 && c < kNumClasses) ? kClasses[c].name : ""; }
 
+// Kernel attributes of every class, once per device in the process, at
+// handle creation (never inside a stream capture)
+static int prepare_kernels(int device) {
+  static std::mutex mu;
+  static std::vector<int> done;
+  std::lock_guard<std::mutex> lk(mu);
+  if (std::find(done.begin(), done.end(), device) != done.end()) return LIBERI_OK;
+  for (int c = 0; c < kNumClasses; ++c)
+    if (int e = prepare_class(c))
+      return fail(LIBERI_EDEVICE, std::string("kernel attributes of ") + kClasses[c].name + ": " +
+                                      hipGetErrorString((hipError_t)e));
+  done.push_back(device);
+  return LIBERI_OK;
+}
+
 int liberi_c_create(liberi_ctx **ctx) {
   if (!ctx) return fail(LIBERI_EARG, "liberi_c_create: NULL handle pointer");
   int ndev = 0;
@@ -271,6 +286,10 @@ int liberi_c_create(liberi_ctx **ctx) {
     delete h;
     return fail(LIBERI_EDEVICE, std::string("liberi_c_create: device is ") + prop.gcnArchName +
                                     ", this library is built for gfx950 (MI355X)");
+  }
+  if (int rc = prepare_kernels(h->device)) {
+    delete h;
+    return rc;
   }
   HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&h->ev0));
@@ -999,8 +1018,13 @@ static int choose_variants(liberi_ctx *h, const BuildArgs &ba, hipStream_t st) {
   for (int c = 0; c < kNumClasses; ++c) {
     ClassState &cs_ = h->cls[c];
     if (!cs_.active || cs_.variant >= 0) continue;
-    const int dflt = (cs_.row_ok && class_kernel_kind(c) == 0) ? 1 : 0;
     const int64_t nq = cs_.dev.q_end - cs_.dev.q_begin;
+    // untimed choice: row tiles for the lane classes, and for the group
+    // classes with a wave-split form unless the class is too small to time
+    const int dflt =
+        (cs_.row_ok && (class_kernel_kind(c) == 0 || (class_has_split(c) && (!h->autotune || nq >= kTuneMinQuartets))))
+            ? 1
+            : 0;
     if (class_has_coop(c)) {
       // group class with a cooperative form (variant 3): one launch of each
       // is cheap at any size, so it is timed without a size threshold

@@ -3174,12 +3174,7 @@ static int launch_row(const ClassDev &cd, const BuildArgs &ba, hipStream_t st) {
     hipLaunchKernelGGL((eri_row_kernel<LA, LB, LC, LD, TR>), grid, dim3(row_threads<LA, LB, LC, LD>()), (unsigned)lds,
                        st, cd, ba);
   } else if constexpr (SplitShape<LA, LB, LC, LD>::exists) {
-    // more than 64 KB of dynamic LDS needs the kernel's limit raised (once
-    // per process; the call is not a stream operation)
-    static const hipError_t attr = hipFuncSetAttribute(
-        reinterpret_cast<const void *>(&eri_split_kernel<LA, LB, LC, LD, TR>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSplitLdsBytes);
-    if (attr != hipSuccess) return attr;
+    // (its dynamic LDS limit was raised by prepare_class at setup)
     hipLaunchKernelGGL((eri_split_kernel<LA, LB, LC, LD, TR>), grid, dim3(SplitShape<LA, LB, LC, LD>::NT),
                        (unsigned)lds, st, cd, ba);
   } else {
@@ -3232,6 +3227,21 @@ static int launch_one(const ClassDev &cd, const BuildArgs &ba, hipStream_t st, i
   return hipGetLastError();
 }
 
+// Kernel attributes a class's launches need, set outside any stream capture
+// (setup): the wave-split kernels take more than 64 KB of dynamic LDS.
+template <int LA, int LB, int LC, int LD>
+static int prepare_one() {
+  if constexpr (SplitShape<LA, LB, LC, LD>::exists) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&eri_split_kernel<LA, LB, LC, LD, false>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSplitLdsBytes);
+    if (e == hipSuccess && RowShape<LA, LB, LC, LD, true>::exists)
+      e = hipFuncSetAttribute(reinterpret_cast<const void *>(&eri_split_kernel<LA, LB, LC, LD, true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSplitLdsBytes);
+    return e;
+  }
+  return hipSuccess;
+}
+
 template <int LA, int LB, int LC, int LD>
 static int64_t scratch_one() {
   if constexpr (use_small<LA, LB, LC, LD>() || Grp<LA, LB, LC, LD>::fits)
@@ -3270,7 +3280,8 @@ static int64_t scratch_one() {
 
 #define DECL_CLASS(id, a, b, cc, d)                                                       \
   int launch_class_##id(const ClassDev &cd, const BuildArgs &ba, hipStream_t st, int nwg); \
-  int64_t scratch_class_##id();
+  int64_t scratch_class_##id();                                                            \
+  int prepare_class_##id();
 LIBERI_CLASS_LIST(DECL_CLASS)
 #undef DECL_CLASS
 
@@ -3278,7 +3289,8 @@ LIBERI_CLASS_LIST(DECL_CLASS)
   int launch_class_##id(const ClassDev &cd, const BuildArgs &ba, hipStream_t st, int nwg) { \
     return launch_one<a, b, cc, d>(cd, ba, st, nwg);                                        \
   }                                                                                         \
-  int64_t scratch_class_##id() { return scratch_one<a, b, cc, d>(); }
+  int64_t scratch_class_##id() { return scratch_one<a, b, cc, d>(); }                      \
+  int prepare_class_##id() { return prepare_one<a, b, cc, d>(); }
 
 #ifndef LIBERI_CLASS_GROUP
 #define LIBERI_CLASS_GROUP -2
@@ -3466,6 +3478,30 @@ int64_t class_row_lds(int c, int transposed, const int32_t *ncol, double xswitch
 #undef X
     default:
       return -1;
+  }
+}
+
+int class_has_split(int c) {
+  switch (c) {
+#define X(id, a, b, cc, d) \
+  case id:                 \
+    return SplitShape<a, b, cc, d>::exists && !use_small<a, b, cc, d>() ? 1 : 0;
+    LIBERI_CLASS_LIST(X)
+#undef X
+    default:
+      return 0;
+  }
+}
+
+int prepare_class(int c) {
+  switch (c) {
+#define X(id, a, b, cc, d) \
+  case id:                 \
+    return prepare_class_##id();
+    LIBERI_CLASS_LIST(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
   }
 }
 

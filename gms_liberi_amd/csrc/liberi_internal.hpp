@@ -126,6 +126,12 @@ struct BuildArgs {
 
 // Launch the kernel of class c over [q_begin, q_end).  Returns hipError_t.
 int launch_class(int c, const ClassDev &cd, const BuildArgs &ba, void *stream, int num_wg_coop);
+// 1 when group class c has the wave-split row-tiled form (its row form)
+int class_has_split(int c);
+// kernel attributes the launches of class c need (the wave-split kernels'
+// dynamic LDS above 64 KB), on the current device; called at setup, outside
+// any stream capture.  Returns hipError_t.
+int prepare_class(int c);
 // doubles of scratch the cooperative kernel of class c needs per workgroup (0 if none)
 int64_t class_scratch_doubles(int c);
 // Rys roots of class c and the end of its tabulated range

@@ -145,8 +145,9 @@ const char *liberi_c_last_error(void);
                                        build. */
 #define LIBERI_OPT_AUTOTUNE 9       /* 1 (default): the first build after setup times direct and
                                        row-tiled digestion per class and keeps the faster; 0: fixed
-                                       choice (row tiles for lane-per-quartet classes).  Setting it
-                                       makes the next build decide again. */
+                                       choice (row tiles for the lane-per-quartet classes and the
+                                       wave-split row tiles for the d classes that have them).
+                                       Setting it makes the next build decide again. */
 #define LIBERI_OPT_GRAPH 10         /* 1 (default): batched form -- the build's class launches run on
                                        four side streams (largest first onto the least-loaded
                                        stream), captured once into a HIP graph that later builds on

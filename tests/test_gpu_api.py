@@ -428,3 +428,30 @@ def test_wave_split_forms_match_group_forms(name):
             assert variant == (1 if diag else orient)
             assert err <= 1e-12 * max(1.0, float(np.max(np.abs(direct))))
         h.destroy()
+
+
+@pytest.mark.gpu
+def test_untimed_first_build_captured_in_a_graph():
+    """With the autotuner off, the first build after setup is captured straight
+    into the graph: the wave-split kernels (their LDS limit raised at handle
+    creation, outside any capture) run from it, and every replay equals the
+    reference."""
+    from gms_liberi_amd import hipmem
+
+    case = Case.load_npz(GOLD / "cfg_ethane.npz")
+    names = liberi.class_names()
+    h = liberi.Handle()
+    h.set_option(liberi.OPT_AUTOTUNE, 0)
+    h.setup(case.basis)
+    n = case.basis.ntri
+    D = hipmem.DeviceArray.from_host(case.density)
+    F = hipmem.DeviceArray(n)
+    s = hipmem.Stream()
+    for _ in range(3):
+        h.fock_build_device(D.address, F.address, n, s.address)
+        s.synchronize()
+        assert np.max(np.abs(F.to_host() - case.fock)) <= 1e-10
+    variants = h.stats()["class_variant"]
+    assert all(variants[names.index(k)] in (1, 2) for k in SPLIT_CLASSES)
+    h.destroy()
+    s.destroy()
