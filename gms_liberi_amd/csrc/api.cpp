@@ -1026,27 +1026,32 @@ static int choose_variants(liberi_ctx *h, const BuildArgs &ba, hipStream_t st) {
             ? 1
             : 0;
     if (class_has_coop(c)) {
-      // group class with a cooperative form (variant 3): one launch of each
-      // is cheap at any size, so it is timed without a size threshold
-      cs_.variant = 0;
+      // group class with a cooperative form (variant 3), and with a
+      // wave-split row form (variant 1) where it has one: one launch of each
+      // is cheap at any size, so they are timed without a size threshold
+      const bool row = cs_.row_ok && cs_.dev.t_end > cs_.dev.t_begin && h->row_tiles != 0;
+      cs_.variant = (dflt == 1 && h->row_tiles != 0) || h->row_tiles == 2 ? 1 : 0;
       cs_.dev.coop = 0;
-      if (!h->autotune || (h->only_class >= 0 && c != h->only_class)) continue;
-      if (const int prev = tuned_choice(c, nq, -3.0); prev >= 0) {
+      if (!h->autotune || h->row_tiles == 2 || (h->only_class >= 0 && c != h->only_class)) continue;
+      if (const int prev = tuned_choice(c, nq, -3.0); prev >= 0 && (prev != 1 || row)) {
         cs_.variant = prev;
         cs_.dev.coop = prev == 3;
         continue;
       }
-      float best[2] = {1e30f, 1e30f};
-      for (int rep = 0; rep < 4; ++rep) {
-        const int v = rep & 1;
+      const int forms[3] = {0, 3, 1};  // group, cooperative, wave-split rows
+      float best[3] = {1e30f, 1e30f, 1e30f};
+      for (int rep = 0; rep < 6; ++rep) {
+        const int k = rep % 3;
+        if (forms[k] == 1 && !row) continue;
         float ms = 0.f;
-        cs_.dev.coop = v;
-        if (int rc = time_class(h, ba, st, c, false, false, ms)) return rc;
-        best[v] = std::min(best[v], ms);
+        cs_.dev.coop = forms[k] == 3;
+        if (int rc = time_class(h, ba, st, c, forms[k] == 1, false, ms)) return rc;
+        best[k] = std::min(best[k], ms);
       }
-      cs_.variant = best[1] < best[0] ? 3 : 0;
+      const int kbest = best[1] < best[0] ? (best[2] < best[1] ? 2 : 1) : (best[2] < best[0] ? 2 : 0);
+      cs_.variant = forms[kbest];
       cs_.dev.coop = cs_.variant == 3;
-      cs_.ms = std::min(best[0], best[1]);
+      cs_.ms = best[kbest];
       set_tuned_choice(c, nq, -3.0, cs_.variant);
       continue;
     }

@@ -1004,7 +1004,9 @@ __host__ __device__ constexpr int split_buffer(int LA, int LB, int LC, int LD, i
 // workgroups) or three (12 waves).
 template <int LA, int LB, int LC, int LD>
 constexpr int split_cone_dirs() {
-  return LA == 2 && LB == 2 ? 1 : 3;  // (dd|dp), (dd|pp): 31 rows e, so two columns per part
+  // (dp|pp), (dp|dp): 16 rows e, four columns per part; the others (25 or
+  // 31 rows) two columns per part
+  return LA == 2 && LB == 1 ? 3 : 1;
 }
 template <int LA, int LB, int LC, int LD>
 constexpr int split_parts() {
@@ -1015,6 +1017,15 @@ constexpr int split_parts() {
   if constexpr (LA == 2 && LB == 1 && LC == 2 && LD == 1) return 6;  // (dp|dp): cone of one d of C
   if constexpr (LA == 2 && LB == 2 && LC == 2 && LD == 1) return 18;  // (dd|dp): one d of C, one direction
   if constexpr (LA == 2 && LB == 2 && LC == 1 && LD == 1) return 9;   // (dd|pp): one p of C, one direction
+  // classes with an f shell (the C-N chain)
+  if constexpr (LA == 2 && LB == 1 && LC == 3 && LD == 0) return 10;  // (dp|fs): one f column
+  if constexpr (LA == 2 && LB == 2 && LC == 3 && LD == 0) return 10;  // (dd|fs): one f column
+  if constexpr (LA == 3 && LB == 1 && LC == 2 && LD == 0) return 6;   // (fp|ds): one d column
+  if constexpr (LA == 3 && LB == 1 && LC == 3 && LD == 0) return 10;  // (fp|fs): one f column
+  if constexpr (LA == 3 && LB == 1 && LC == 1 && LD == 1) return 9;   // (fp|pp): one p of C, one direction
+  if constexpr (LA == 3 && LB == 1 && LC == 2 && LD == 1) return 18;  // (fp|dp): one d of C, one direction
+  if constexpr (LA == 2 && LB == 2 && LC == 3 && LD == 1) return 30;  // (dd|fp): one f of C, one direction
+  if constexpr (LA == 3 && LB == 1 && LC == 3 && LD == 1) return 30;  // (fp|fp): one f of C, one direction
   return 0;
 }
 template <int LA, int LB, int LC, int LD>
@@ -1024,6 +1035,8 @@ constexpr int split_waves() {
   if constexpr (LA == 2 && LB == 1 && LC == 2 && LD == 1) return 2;  // (dp|dp): 3 parts per wave
   if constexpr (LA == 2 && LB == 2 && LC == 2 && LD == 1) return 2;  // (dd|dp): 9 parts per wave
   if constexpr (LA == 2 && LB == 2 && LC == 1 && LD == 1) return 1;  // (dd|pp): 9 parts per wave
+  if constexpr (LA == 3 && LB == 1 && LC == 1 && LD == 1) return 1;  // (fp|pp): 9 parts per wave
+  if constexpr ((LA == 3 || LC == 3) && S % 2 == 0) return 2;  // f classes: S/2 parts per wave
   return S;
 }
 
