@@ -2171,7 +2171,14 @@ constexpr int grp_pad(int map, int p, int nq, int g, int nr, int gsz0, int bufp,
 // classes whose group form runs the HRR in registers (Grp::REG): one
 // column of (e0|f0) and one row (ab| of the ket phase per lane
 __host__ __device__ constexpr bool grp_reg_hrr(int LA, int LB, int LC, int LD) {
-  return LA == 2 && LB == 2 && LC == 2 && LD == 2;
+  // (dd|dd), and the (df| classes with a ket HRR or an f ket shell but
+  // (df|ff) (46-row columns; rows of at most 46 columns and two ket levels).
+  // Measured on the C-N chain against the level-by-level form: (df|df) 2.41
+  // -> 1.11 ms, (df|dd) 2.36 -> 1.32, (df|dp) 1.35 -> 1.03, (df|fs) 0.93 ->
+  // 0.84, (df|pp) 0.46 -> 0.41; (df|ds) 1.09 -> 1.34 and (df|ps) 0.72 ->
+  // 1.29 stay level by level
+  return (LA == 2 && LB == 2 && LC == 2 && LD == 2) ||
+         (LA == 3 && LB == 2 && LC >= 1 && LD <= 2 && LC >= LD && (LD > 0 || LC == 3));
 }
 
 template <int LA, int LB, int LC, int LD>
@@ -2305,7 +2312,7 @@ __device__ __forceinline__ void grp_run(const ClassDev &cd, const BuildArgs &ba,
   constexpr int GSZ = S::GSZ;  // stride of the 2-D tables (padded against bank conflicts)
   constexpr int RT = S::RT, REC = S::REC, KT = S::KT;
   static_assert(RT >= KT && KT * S::GS <= S::UNI, "no room for the tables and rule records");
-  static_assert(!S::REG || (S::NAB <= G && LB > 0 && 2 * S::BUFP <= S::UNI), "register HRR: one row ab per lane");
+  static_assert(!S::REG || (LB > 0 && 2 * S::BUFP <= S::UNI), "register HRR: layout");
 
   // lanes past the NQ groups (G need not divide 64) idle: their gl lies past
   // every loop bound and their slots are never valid
@@ -2558,26 +2565,30 @@ __device__ __forceinline__ void grp_run(const ClassDev &cd, const BuildArgs &ba,
         for (int ab = 0; ab < NAB; ++ab) bufT[ab * TS + f] = (LB % 2) ? c1[ab] : c0[ab];
       }
       wave_sync();
-      // ket HRR: one row ab per lane, in registers, then the cartesian
-      // scales and the value cutoff
-      const bool row = valid && gl < NAB;
-      double v[NCD];
-      if (row) {
-        double r0[S::RBUF], r1[S::RBUF];
+      // ket HRR: rows ab = gl, gl + G, ... of the lane, in registers, then
+      // the cartesian scales and the value cutoff
+      constexpr int NJ = (NAB + G - 1) / G;
+      double v[NJ][NCD];
 #pragma unroll
-        for (int f = 0; f < NF; ++f) r0[f] = bufT[gl * TS + f];
-        hrr_ket<LC, LD, 1>(r0, r1, ket.ABx, ket.ABy, ket.ABz);
-        constexpr Range<LC, LC> rc{};
-        constexpr Range<LD, LD> rd{};
-        const double sab = scale_rt(LA, gl / NB) * scale_rt(LB, gl % NB);
+      for (int k = 0; k < NJ; ++k) {
+        const int r = gl + k * G;
+        if (valid && r < NAB) {
+          double r0[S::RBUF], r1[S::RBUF];
 #pragma unroll
-        for (int c = 0; c < NC; ++c)
+          for (int f = 0; f < NF; ++f) r0[f] = bufT[r * TS + f];
+          hrr_ket<LC, LD, 1>(r0, r1, ket.ABx, ket.ABy, ket.ABz);
+          constexpr Range<LC, LC> rc{};
+          constexpr Range<LD, LD> rd{};
+          const double sab = scale_rt(LA, r / NB) * scale_rt(LB, r % NB);
 #pragma unroll
-          for (int d = 0; d < ND; ++d) {
-            const double x = ((LD % 2) ? r1[c * ND + d] : r0[c * ND + d]) * sab *
-                             (cart_scale(rc.lx[c], rc.ly[c], rc.lz[c]) * cart_scale(rd.lx[d], rd.ly[d], rd.lz[d]));
-            v[c * ND + d] = fabs(x) < ba.value_cut ? 0.0 : x;
-          }
+          for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+              const double x = ((LD % 2) ? r1[c * ND + d] : r0[c * ND + d]) * sab *
+                               (cart_scale(rc.lx[c], rc.ly[c], rc.lz[c]) * cart_scale(rd.lx[d], rd.ly[d], rd.lz[d]));
+              v[k][c * ND + d] = fabs(x) < ba.value_cut ? 0.0 : x;
+            }
+        }
       }
       wave_sync();  // [ab][f] read: its space takes the partial sums
       // digestion: each lane's row ab contributes to the six blocks; the
@@ -2593,22 +2604,26 @@ __device__ __forceinline__ void grp_run(const ClassDev &cd, const BuildArgs &ba,
         jab_a = bra.bfA;
         jab_b = bra.bfB;
       }
-      if (row) {
-        const int a = gl / NB, b = gl - a * NB;
+#pragma unroll
+      for (int k = 0; k < NJ; ++k) {
+        const int r = gl + k * G;
+        if (!(valid && r < NAB)) continue;
+        const double *w = v[k];
+        const int a = r / NB, b = r - a * NB;
         double t = 0.0;
 #pragma unroll
-        for (int k = 0; k < NCD; ++k) t += v[k] * DCD[k];
-        jab[0] += 4.0 * s * t;  // J(AB): the row's own element, in registers over the bra pair's run
-        const double dab = DAB[gl];
+        for (int i = 0; i < NCD; ++i) t += w[i] * DCD[i];
+        jab[k] += 4.0 * s * t;  // J(AB): the row's own element, in registers over the bra pair's run
+        const double dab = DAB[r];
 #pragma unroll
-        for (int k = 0; k < NCD; ++k) JC[gl * JS + k] = v[k] * dab;
+        for (int i = 0; i < NCD; ++i) JC[r * JS + i] = w[i] * dab;
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
           double kac = 0.0, kbc = 0.0;
 #pragma unroll
           for (int d = 0; d < ND; ++d) {
-            kac += v[c * ND + d] * DBD[b * ND + d];
-            kbc += v[c * ND + d] * DAD[a * ND + d];
+            kac += w[c * ND + d] * DBD[b * ND + d];
+            kbc += w[c * ND + d] * DAD[a * ND + d];
           }
           KAC[(a * NB + b) * NC + c] = kac;
           KBC[(b * NA + a) * NC + c] = kbc;
@@ -2618,8 +2633,8 @@ __device__ __forceinline__ void grp_run(const ClassDev &cd, const BuildArgs &ba,
           double kbd = 0.0, kad = 0.0;
 #pragma unroll
           for (int c = 0; c < NC; ++c) {
-            kbd += v[c * ND + d] * DAC[a * NC + c];
-            kad += v[c * ND + d] * DBC[b * NC + c];
+            kbd += w[c * ND + d] * DAC[a * NC + c];
+            kad += w[c * ND + d] * DBC[b * NC + c];
           }
           KBD[(b * NA + a) * ND + d] = kbd;
           KAD[(a * NB + b) * ND + d] = kad;
@@ -2887,7 +2902,7 @@ __device__ __forceinline__ void grp_run(const ClassDev &cd, const BuildArgs &ba,
 }
 
 template <int LA, int LB, int LC, int LD>
-__attribute__((amdgpu_waves_per_eu(grp_reg_hrr(LA, LB, LC, LD)                                              ? 2
+__attribute__((amdgpu_waves_per_eu(grp_reg_hrr(LA, LB, LC, LD) && LA + LB + LC + LD == 8               ? 2
                                    : LA <= 2 && LB <= 2 && LC <= 2 && LD <= 2 && LA + LB + LC + LD <= 7 ? 3
                                                                                                         : 1,
                                    8)))
