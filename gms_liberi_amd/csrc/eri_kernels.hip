@@ -710,6 +710,61 @@ __host__ __device__ inline int64_t row_columns(const int32_t *ncol) {
   return (int64_t)ncol[R::M1] + (R::M1 == R::M2 ? 0 : ncol[R::M2]);
 }
 
+// x plus the value of lane CTRL-shifted x (DPP), lanes outside the row or
+// masked off adding 0
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_add(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, ROW_MASK, 0xf, false);
+  return x + __hiloint2double(hi, lo);
+}
+// Sums over the 64 lanes of a wave of K values per lane, added to LDS:
+// dst[idx(k)] += sum over lanes of x[k].  Two halvings first exchange whole
+// half-waves and rows between the registers of two values
+// (v_permlane32_swap: upper 32 lanes of the first with the lower 32 of the
+// second; v_permlane16_swap: odd rows of the first with even rows of the
+// second), so one add leaves value j in one half (row) and value j + h in
+// the other: the number of values a lane carries halves at each step.  Row
+// r then holds values (r & 1 ? K2 + j : j) + (r >> 1 ? K1 : 0), j < K2,
+// each summed over its four lanes of the row position; prefix doubling
+// inside the row (row_shr 1, 2, 4, 8) completes the sums in lane 15 of the
+// row, which adds its K2 totals to LDS (four rows, four distinct addresses).
+__device__ __forceinline__ double halves_add(double x, double y) {
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(x), (unsigned)__double2loint(y), false,
+                                                   false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(x), (unsigned)__double2hiint(y), false,
+                                                   false);
+  return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
+}
+__device__ __forceinline__ double rows_add(double x, double y) {
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(x), (unsigned)__double2loint(y), false,
+                                                   false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(x), (unsigned)__double2hiint(y), false,
+                                                   false);
+  return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
+}
+template <int K, class Idx>
+__device__ __forceinline__ void wave_sum_to_lds(const double (&x)[K], double *dst, Idx idx) {
+  constexpr int K1 = (K + 1) / 2, K2 = (K1 + 1) / 2;
+  double a[K1], b[K2];
+#pragma unroll
+  for (int j = 0; j < K1; ++j) a[j] = halves_add(x[j], j + K1 < K ? x[j + K1] : 0.0);
+#pragma unroll
+  for (int j = 0; j < K2; ++j) b[j] = rows_add(a[j], j + K2 < K1 ? a[j + K2] : 0.0);
+  const int lane = (int)__lane_id(), r = lane >> 4;
+#pragma unroll
+  for (int j = 0; j < K2; ++j) {
+    double t = b[j];
+    t = dpp_add<0x111, 0xf>(t);
+    t = dpp_add<0x112, 0xf>(t);
+    t = dpp_add<0x114, 0xf>(t);
+    t = dpp_add<0x118, 0xf>(t);
+    const int s = (r & 1) ? K2 + j : j;
+    const int k = s + ((r >> 1) ? K1 : 0);
+    if ((lane & 15) == 15 && s < K1 && k < K && t != 0.0) lds_add(&dst[idx(k)], t);
+  }
+}
+
 // Row-tiled lane-per-quartet kernel.  One workgroup owns one tile: a run of
 // quartets that share the fixed pair.  Four of the six digestion blocks --
 // the exchange blocks K(AC), K(AD), K(BC), K(BD) -- land in Fock rows of the
@@ -935,13 +990,7 @@ __global__ __launch_bounds__((row_threads<LA, LB, LC, LD>())) void eri_row_kerne
 
   count_fp32(ba, nlow);
   // the fixed pair's Coulomb block: reduce over each wave, then over the waves
-#pragma unroll
-  for (int k = 0; k < NFIX; ++k) {
-    double x = jfix[k];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-    if ((threadIdx.x & 63) == 0 && x != 0.0) lds_add(&jfix_s[k], x);
-  }
+  wave_sum_to_lds<NFIX>(jfix, jfix_s, [](int k) { return k; });
   __syncthreads();
   for (int i = threadIdx.x; i < NFIX; i += blockDim.x) {
     const double x = jfix_s[i];
@@ -1209,27 +1258,6 @@ __device__ __forceinline__ void split_integrals(const PairRec &bra, const PairRe
         }
 }
 
-// x plus the value of lane CTRL-shifted x (DPP), lanes outside the row or
-// masked off adding 0
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ double dpp_add(double x) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, ROW_MASK, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, ROW_MASK, 0xf, false);
-  return x + __hiloint2double(hi, lo);
-}
-// Sum over the 64 lanes of a wave, complete in lane 63: prefix doubling
-// inside each row of 16 lanes (row_shr 1, 2, 4, 8), then the row sums
-// carried over by row_bcast:15 and row_bcast:31
-__device__ __forceinline__ double wave_sum_dpp(double x) {
-  x = dpp_add<0x111, 0xf>(x);
-  x = dpp_add<0x112, 0xf>(x);
-  x = dpp_add<0x114, 0xf>(x);
-  x = dpp_add<0x118, 0xf>(x);
-  x = dpp_add<0x142, 0xa>(x);
-  x = dpp_add<0x143, 0xc>(x);
-  return x;
-}
-
 constexpr int64_t kSplitLdsBytes = 160 * 1024;
 
 // Quartets of one part (slice PART) of a wave-split row tile: the run of 64
@@ -1421,14 +1449,11 @@ __device__ __forceinline__ void split_part(const ClassDev &cd, const BuildArgs &
           }
       }
     }
-    // this part's share of the fixed pair's Coulomb block: one wave sum per
-    // element (every lane takes part; lanes past the tile hold zeros)
-#pragma unroll
-    for (int k = 0; k < NJ; ++k) {
-      const double x = wave_sum_dpp(jf[k]);
-      const int fk = TR ? (C0 + k / NDP) * ND + D0 + k % NDP : (A0 + k / NB) * NB + k % NB;
-      if (lane == 63 && x != 0.0) lds_add(&jfix_s[fk], x);
-    }
+    // this part's share of the fixed pair's Coulomb block, summed over the
+    // wave (every lane takes part; lanes past the tile hold zeros)
+    wave_sum_to_lds<NJ>(jf, jfix_s, [](int k) {
+      return TR ? (C0 + k / NDP) * ND + D0 + k % NDP : (A0 + k / NB) * NB + k % NB;
+    });
   }
 }
 
