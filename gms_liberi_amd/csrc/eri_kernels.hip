@@ -837,10 +837,10 @@ __global__ __launch_bounds__((row_threads<LA, LB, LC, LD>())) void eri_row_kerne
   __syncthreads();
 
   unsigned nlow = 0;  // quartets this lane evaluated in FP32
+  int sg = seg0;      // segment of the lane's quartet (its quartets ascend)
   for (int64_t qb = q0; qb < q1; qb += blockDim.x) {
     const int64_t q = qb + threadIdx.x;
     if (q >= q1) continue;
-    int sg = seg0;
     while (sg + 1 < cd.nseg && cd.seg_start[sg + 1] <= q) ++sg;
     const int32_t imov = cd.ket_list[cd.seg_ket0[sg] + (int32_t)(q - cd.seg_start[sg])];
     const PairRec mov = ba.pairs[imov];
