@@ -368,7 +368,8 @@ def main():
                     "total_tflops": round(ge["total_flops"] / (ge["ms_per_step"] * 1e-3) * 1e-12, 3),
                     "accuracy_vs_reference": {"max_abs_dF": float(np.max(np.abs(ge["fock"] - ce.fock))),
                                               "dE_2e": float(0.5 * np.dot(ce.density, ge["fock"] - ce.fock))},
-                    "top_kernels": dict(list(ge["per_class"].items())[:8])}
+                    "top_kernels": dict(list(ge["per_class"].items())[:8]),
+                    "forms": {k: v["form"] for k, v in ge["per_class"].items()}}
         if not a.no_cpu:
             threads = len(os.sched_getaffinity(0))
             r = ref_builds(ce, threads, 1)
