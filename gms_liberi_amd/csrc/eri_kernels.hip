@@ -93,9 +93,12 @@ __host__ __device__ constexpr double cart_scale(int lx, int ly, int lz) {
   return 1.0;
 }
 
-// packed lower-triangle index of (i, j); i, j < kMaxBasis keeps i(i+1) below 2^32
+// packed lower-triangle index of (i, j); i, j < kMaxBasis keeps i(i+1) below
+// 2^32, and the 24-bit multiply (full rate, where a 32-bit one is a quarter)
+// holds i + 1 < 2^24
 __device__ __forceinline__ uint32_t tri(uint32_t i, uint32_t j) {
-  return i >= j ? ((i * (i + 1)) >> 1) + j : ((j * (j + 1)) >> 1) + i;
+  const uint32_t m = i >= j ? i : j, n = i >= j ? j : i;
+  return (__umul24(m, m + 1) >> 1) + n;
 }
 
 // Reciprocal and reciprocal square root for the primitive loops: the
@@ -132,7 +135,7 @@ __device__ __forceinline__ void rys_rule(double X, double xswitch, double (&u)[N
   if (X < xswitch) {
     const int j = (int)(X * RYS_INV_H);
     const double s = X * (2.0 * RYS_INV_H) - (double)(2 * j + 1);
-    const double *c = tab + j * istride;
+    const double *c = tab + __mul24(j, istride);
 #pragma unroll
     for (int r = 0; r < 2 * N; ++r) {
       const double *cr = c + r * (RYS_DEG + 1);
