@@ -372,7 +372,7 @@ int liberi_c_set_option(liberi_ctx *h, int32_t opt, double v) {
       for (int i = 0; i < kNumTileLens; ++i)
         if (v == (double)kTileLens[i]) k = i;
       if (v != 0.0 && k < 0)
-        return fail(LIBERI_EARG, "liberi_c_set_option: TILE_LEN must be 0, 1024, 4096 or 16384");
+        return fail(LIBERI_EARG, "liberi_c_set_option: TILE_LEN must be 0, 512, 1024, 4096 or 16384");
       h->tile_len = v == 0.0 ? -1 : k;
       for (auto &c : h->cls) c.tile = -1;  // chosen again at the next build
       break;

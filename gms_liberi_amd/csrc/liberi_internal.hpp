@@ -164,9 +164,9 @@ int launch_group_fused(const ClassDev *cds, const BuildArgs &ba, const FusedItem
 // longest row tile, quartets
 // row-tile lengths (quartets of one row per workgroup) the autotuner times
 // per row-tiled class; kDefaultTile where it does not time
-constexpr int kNumTileLens = 3;
-constexpr int64_t kTileLens[kNumTileLens] = {1024, 4096, 16384};
-constexpr int kDefaultTile = 1;
+constexpr int kNumTileLens = 4;
+constexpr int64_t kTileLens[kNumTileLens] = {512, 1024, 4096, 16384};
+constexpr int kDefaultTile = 2;
 // LDS the row-tiled kernel may take per workgroup
 constexpr int64_t kRowLdsBytes = 64 * 1024;
 // workgroups the cooperative kernels use

@@ -166,7 +166,7 @@ const char *liberi_c_last_error(void);
                                         launch per class.  Read at each build. */
 #define LIBERI_OPT_TILE_LEN 14       /* row-tile length of the row-tiled classes, in quartets of one row
                                         per workgroup: 0 (default) timed per class at the first build
-                                        (LIBERI_OPT_AUTOTUNE; 4096 where untimed), or 1024, 4096, 16384
+                                        (LIBERI_OPT_AUTOTUNE; 4096 where untimed), or 512, 1024, 4096, 16384
                                         for every class.  Read at each build. */
 #define LIBERI_OPT_ROW_ORIENT 15     /* orientation of the row tiles of a non-diagonal class: 0 (default)
                                         rows over the pair whose Coulomb block is larger, among the
@@ -203,8 +203,8 @@ typedef struct {
   int32_t class_fp32[LIBERI_NCLASS];    /* 1: the mixed form (LIBERI_OPT_FP32_TAU > 0) evaluates this
                                            class's qualifying quartets in FP32, chosen per class from
                                            measured time at the first build under that tau; 0: FP64 */
-  int32_t class_tile[LIBERI_NCLASS];    /* row-tile length (quartets of one row per workgroup: 1024,
-                                           4096 or 16384) of a row-tiled class, timed per class at the
+  int32_t class_tile[LIBERI_NCLASS];    /* row-tile length (quartets of one row per workgroup: 512,
+                                           1024, 4096 or 16384) of a row-tiled class, timed per class at the
                                            first build after setup (LIBERI_OPT_AUTOTUNE) or 4096;
                                            0 where the class is not row-tiled */
 } liberi_stats;

@@ -370,7 +370,7 @@ def test_row_tile_lengths_give_the_same_fock():
     h = liberi.Handle()
     h.setup(case.basis)
     out = {}
-    for length in (0, 1024, 4096, 16384):
+    for length in (0, 512, 1024, 4096, 16384):
         h.set_option(liberi.OPT_TILE_LEN, length)
         assert h.get_option(liberi.OPT_TILE_LEN) == length
         out[length] = h.fock_build(case.density)
@@ -381,7 +381,7 @@ def test_row_tile_lengths_give_the_same_fock():
         if length:
             assert np.all(st["class_tile"][row] == length)
         else:
-            assert set(st["class_tile"][row].tolist()) <= {1024, 4096, 16384}
+            assert set(st["class_tile"][row].tolist()) <= {512, 1024, 4096, 16384}
             print("tile lengths chosen:", {liberi.class_names()[c]: int(st["class_tile"][c])
                                            for c in np.flatnonzero(row)})
     with pytest.raises(liberi.LiberiError):
