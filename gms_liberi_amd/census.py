@@ -5,7 +5,8 @@ gms_libERI, in that order.  ``CLASS_L`` gives (la, lb, lc, ld) with the
 higher-L shell first in each pair, as the HIP kernels evaluate them.
 
 FLOP model (FMA = 2, add/sub/mul/div/sqrt = 1), per primitive quartet:
-  setup 15 + Rys rule 2*nr*(2*RYS_DEG) + per root [22 + 3 G tables + 3*ne*nf]
+  setup 15 + Rys rule 2*nr*(2*deg) + per root [22 + 3 G tables + 3*ne*nf]
+(deg: the table's polynomial degree for nr roots, 7 for nr = 1, else 9)
 per contracted quartet:
   horizontal recurrence 2 per produced element + scale/cutoff 2*nabcd +
   digestion 12*nabcd (six block contractions).
@@ -32,6 +33,11 @@ CLASS_L = [(0, 0, 0, 0), (0, 0, 1, 0), (0, 0, 1, 1), (1, 0, 1, 1), (1, 0, 1, 0),
            (3, 3, 1, 0), (3, 3, 1, 1), (3, 3, 2, 0), (3, 3, 2, 1), (3, 3, 2, 2), (3, 3, 3, 2)]
 
 RYS_DEG = 9
+RYS_DEG_SMALL, RYS_NSMALL = 7, 1  # the one-root table
+
+
+def rys_deg(nr: int) -> int:
+    return RYS_DEG_SMALL if nr <= RYS_NSMALL else RYS_DEG
 
 
 def ncart(l: int) -> int:
@@ -56,7 +62,7 @@ def class_params(c: int) -> dict:
 def flops_per_prim_quartet(c: int) -> int:
     p = class_params(c)
     g = 3 * (p["ni"] - 1) + 5 * p["ni"] * (p["nk"] - 1)  # one 2-D table
-    return 15 + 2 * p["nr"] * 2 * RYS_DEG + p["nr"] * (22 + 3 * g + 3 * p["ne"] * p["nf"])
+    return 15 + 2 * p["nr"] * 2 * rys_deg(p["nr"]) + p["nr"] * (22 + 3 * g + 3 * p["ne"] * p["nf"])
 
 
 def flops_per_quartet(c: int) -> int:

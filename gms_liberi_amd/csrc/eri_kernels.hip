@@ -31,6 +31,12 @@ static const double rys_xmax_host[RYS_NMAX + 1] = {0.0, 39.0, 46.5, 53.5, 60.0, 
 
 namespace liberi {
 
+// polynomial degree, inverse interval width and intervals of the table of
+// the N-root rule (the one-root rule has its own, finer grid: RYS_NSMALL)
+__host__ __device__ constexpr int rys_deg(int n) { return n <= RYS_NSMALL ? RYS_DEG_SMALL : RYS_DEG; }
+__host__ __device__ constexpr double rys_inv_h(int n) { return n <= RYS_NSMALL ? RYS_INV_H_SMALL : RYS_INV_H; }
+__host__ __device__ constexpr int rys_ncoef(int n) { return 2 * n * (rys_deg(n) + 1); }
+
 // ------------------------------------------------------------ cartesian maps
 __host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 __host__ __device__ constexpr int nrange(int l0, int l1) {
@@ -131,17 +137,19 @@ __device__ __forceinline__ float nr_rcp(float x) {
 template <int N>
 __device__ __forceinline__ void rys_rule(double X, double xswitch, double (&u)[N], double (&w)[N],
                                          const double *tab = rys_table + rys_offset[N],
-                                         int istride = 2 * N * (RYS_DEG + 1)) {
+                                         int istride = rys_ncoef(N)) {
   if (X < xswitch) {
-    const int j = (int)(X * RYS_INV_H);
-    const double s = X * (2.0 * RYS_INV_H) - (double)(2 * j + 1);
+    constexpr int DG = rys_deg(N);
+    constexpr double IH = rys_inv_h(N);
+    const int j = (int)(X * IH);
+    const double s = X * (2.0 * IH) - (double)(2 * j + 1);
     const double *c = tab + __mul24(j, istride);
 #pragma unroll
     for (int r = 0; r < 2 * N; ++r) {
-      const double *cr = c + r * (RYS_DEG + 1);
-      double acc = cr[RYS_DEG];
+      const double *cr = c + r * (DG + 1);
+      double acc = cr[DG];
 #pragma unroll
-      for (int k = RYS_DEG - 1; k >= 0; --k) acc = fma(acc, s, cr[k]);
+      for (int k = DG - 1; k >= 0; --k) acc = fma(acc, s, cr[k]);
       if (r < N)
         u[r] = acc;
       else
@@ -163,13 +171,15 @@ __device__ __forceinline__ void rys_rule(double X, double xswitch, double (&u)[N
 template <int N>
 __device__ __forceinline__ void rys_root(double X, double xswitch, int r, double &u, double &w) {
   if (X < xswitch) {
-    const int j = (int)(X * RYS_INV_H);
-    const double s = X * (2.0 * RYS_INV_H) - (double)(2 * j + 1);
-    const double *c = rys_table + rys_offset[N] + j * (2 * N * (RYS_DEG + 1));
-    const double *cu = c + r * (RYS_DEG + 1), *cw = c + (N + r) * (RYS_DEG + 1);
-    double au = cu[RYS_DEG], aw = cw[RYS_DEG];
+    constexpr int DG = rys_deg(N);
+    constexpr double IH = rys_inv_h(N);
+    const int j = (int)(X * IH);
+    const double s = X * (2.0 * IH) - (double)(2 * j + 1);
+    const double *c = rys_table + rys_offset[N] + j * rys_ncoef(N);
+    const double *cu = c + r * (DG + 1), *cw = c + (N + r) * (DG + 1);
+    double au = cu[DG], aw = cw[DG];
 #pragma unroll
-    for (int k = RYS_DEG - 1; k >= 0; --k) {
+    for (int k = DG - 1; k >= 0; --k) {
       au = fma(au, s, cu[k]);
       aw = fma(aw, s, cw[k]);
     }
@@ -406,7 +416,7 @@ __device__ __forceinline__ void small_integrals(const PairRec &bra, const PairRe
                                                 double xswitch, double cut,
                                                 double (&v)[Small<LA, LB, LC, LD>::NABCD],
                                                 const double *tab = rys_table + rys_offset[Small<LA, LB, LC, LD>::NR],
-                                                int istride = 2 * Small<LA, LB, LC, LD>::NR * (RYS_DEG + 1)) {
+                                                int istride = rys_ncoef(Small<LA, LB, LC, LD>::NR)) {
   using S = Small<LA, LB, LC, LD>;
   constexpr int NE = S::NE, NF = S::NF, NR = S::NR, NI = S::LAB + 1, NK = S::LCD + 1;
   constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND;
@@ -680,13 +690,14 @@ __device__ __forceinline__ void lds_add(double *p, double v) {
 // pads each interval to an odd stride, so lanes reading different intervals
 // spread over the LDS banks.
 __host__ __device__ inline int64_t rys_table_intervals(int n, double xswitch) {
-  int64_t nint = (int64_t)ceil(xswitch * RYS_INV_H);
-  const int64_t cap = n == 1 ? 78 : n == 2 ? 93 : n == 3 ? 107 : n == 4 ? 120 : n == 5 ? 132 : n == 6 ? 144 : 155;
+  int64_t nint = (int64_t)ceil(xswitch * rys_inv_h(n));
+  const double xmax = n == 1 ? 39.0 : n == 2 ? 46.5 : n == 3 ? 53.5 : n == 4 ? 60.0 : n == 5 ? 66.0 : n == 6 ? 72.0 : 77.5;
+  const int64_t cap = (int64_t)(xmax * rys_inv_h(n) + 0.5);
   if (nint > cap) nint = cap;
   if (nint < 0) nint = 0;
   return nint;
 }
-__host__ __device__ constexpr int rys_lds_stride(int n) { return 2 * n * (RYS_DEG + 1) + 1; }
+__host__ __device__ constexpr int rys_lds_stride(int n) { return rys_ncoef(n) + 1; }
 __host__ __device__ inline int64_t rys_table_used(int n, double xswitch) {
   return rys_table_intervals(n, xswitch) * rys_lds_stride(n);
 }
@@ -810,7 +821,7 @@ __global__ __launch_bounds__((row_threads<LA, LB, LC, LD>())) void eri_row_kerne
   // intervals padded to an odd stride
   double *tab = rows + NROW * ncols;
   {
-    constexpr int IN = 2 * S::NR * (RYS_DEG + 1), LS = rys_lds_stride(S::NR);
+    constexpr int IN = rys_ncoef(S::NR), LS = rys_lds_stride(S::NR);
     const int ntab = (int)rys_table_intervals(S::NR, cd.xswitch) * IN;
     const double *src = rys_table + rys_offset[S::NR];
     for (int i = threadIdx.x; i < ntab; i += blockDim.x) tab[(i / IN) * LS + i % IN] = src[i];
@@ -1499,7 +1510,7 @@ __global__ __launch_bounds__((SplitShape<LA, LB, LC, LD>::NT)) void eri_split_ke
   const int off2 = R::M1 == R::M2 ? 0 : ba.ncol[R::M1];
   double *tab = rows + NROW * ncols;
   {
-    constexpr int IN = 2 * P::NR * (RYS_DEG + 1), LS = rys_lds_stride(P::NR);
+    constexpr int IN = rys_ncoef(P::NR), LS = rys_lds_stride(P::NR);
     const int ntab = (int)rys_table_intervals(P::NR, cd.xswitch) * IN;
     const double *src = rys_table + rys_offset[P::NR];
     for (int i = threadIdx.x; i < ntab; i += blockDim.x) tab[(i / IN) * LS + i % IN] = src[i];

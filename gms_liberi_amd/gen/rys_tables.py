@@ -8,9 +8,9 @@ Roots u_i(X) and weights w_i(X) are computed here in high precision
 (mpmath): moments F_0..F_{2n-1}, Chebyshev's moment algorithm for the
 three-term recurrence, Golub-Welsch eigen-decomposition of the Jacobi matrix.
 
-On [0, X_n) they are stored as piecewise polynomials: intervals of width H,
-per interval a polynomial of degree DEG in s = (X - x_mid) / (H / 2) in
-[-1, 1] for every u_i and w_i (fitted at Chebyshev nodes, then converted to
+On [0, X_n) they are stored as piecewise polynomials: intervals of width H
+(H_SMALL for n <= NSMALL), per interval a polynomial of degree DEG (DEG_SMALL)
+in s = (X - x_mid) / (H / 2) in [-1, 1] for every u_i and w_i (fitted at Chebyshev nodes, then converted to
 the power basis so the kernels evaluate with Horner).  For X >= X_n the
 asymptotic rule is exact to double precision: with h_i, W_i the positive
 Gauss-Hermite nodes/weights of order 2n,  u_i = h_i^2 / X,  w_i = W_i / sqrt(X).
@@ -34,7 +34,21 @@ import numpy as np
 NMAX = 7
 H = 0.5          # interval width in X
 DEG = 9          # polynomial degree per interval
+# the one-root rule (the (ss|ss) and (ss|ps) classes, whose kernels read the
+# table the most) on intervals of H_SMALL with degree DEG_SMALL: 20 % fewer
+# coefficients per evaluation for the same accuracy
+NSMALL = 1
+H_SMALL = 0.25
+DEG_SMALL = 7
 DPS = 60
+
+
+def h_of(n: int) -> float:
+    return H_SMALL if n <= NSMALL else H
+
+
+def deg_of(n: int) -> int:
+    return DEG_SMALL if n <= NSMALL else DEG
 
 OUT = Path(__file__).resolve().parents[1] / "csrc" / "rys_table.inc"
 
@@ -109,26 +123,27 @@ def find_xmax(n: int) -> float:
 def fit_interval(args):
     n, j = args
     mp.mp.dps = DPS
-    x0 = j * H
-    xm = x0 + H / 2
+    h, deg = h_of(n), deg_of(n)
+    x0 = j * h
+    xm = x0 + h / 2
     # Chebyshev nodes of the first kind on [-1, 1]
-    m = DEG + 1
+    m = deg + 1
     s = [math.cos(math.pi * (k + 0.5) / m) for k in range(m)]
     vals = np.zeros((2 * n, m))
     for k, sk in enumerate(s):
-        u, w = rys_rule(n, mp.mpf(xm) + mp.mpf(sk) * H / 2)
+        u, w = rys_rule(n, mp.mpf(xm) + mp.mpf(sk) * h / 2)
         vals[:n, k] = [float(v) for v in u]
         vals[n:, k] = [float(v) for v in w]
     # interpolate, convert to power basis in s
     coefs = np.zeros((2 * n, m))
-    V = np.polynomial.chebyshev.chebvander(np.array(s), DEG)
+    V = np.polynomial.chebyshev.chebvander(np.array(s), deg)
     for r in range(2 * n):
         c = np.linalg.solve(V, vals[r])
         coefs[r] = np.polynomial.chebyshev.cheb2poly(c)
     # check at off-node points against the exact rule, evaluating as the kernel does
     err = 0.0
     for sk in np.linspace(-1, 1, 7):
-        u, w = rys_rule(n, mp.mpf(xm) + mp.mpf(float(sk)) * H / 2)
+        u, w = rys_rule(n, mp.mpf(xm) + mp.mpf(float(sk)) * h / 2)
         ex = [float(v) for v in u] + [float(v) for v in w]
         for r in range(2 * n):
             acc = 0.0
@@ -144,7 +159,7 @@ def main():
     for n in range(1, NMAX + 1):
         xmax[n] = find_xmax(n)
         print(f"n={n}: asymptotic from X={xmax[n]}", flush=True)
-    jobs = [(n, j) for n in range(1, NMAX + 1) for j in range(int(round(xmax[n] / H)))]
+    jobs = [(n, j) for n in range(1, NMAX + 1) for j in range(int(round(xmax[n] / h_of(n))))]
     res = {}
     worst = {}
     with ProcessPoolExecutor(max_workers=os.cpu_count()) as ex:
@@ -159,19 +174,24 @@ def main():
              "// Rys roots u_i = t_i^2 and weights w_i for n = 1..%d roots." % NMAX,
              "// Piecewise: interval j covers X in [j*H, (j+1)*H); value_r(X) = sum_k c[k] s^k,",
              "// s = (X - (j+0.5)*H) / (H/2).  Per interval the 2n rows are u_1..u_n, w_1..w_n",
-             "// (roots ascending), each RYS_DEG+1 coefficients c[0..RYS_DEG].",
+             "// (roots ascending), each DEG+1 coefficients c[0..DEG]; H = RYS_H and DEG = RYS_DEG,",
+             "// except for n <= RYS_NSMALL: H = RYS_H_SMALL, DEG = RYS_DEG_SMALL.",
              "// For X >= rys_xmax[n]: u_i = rys_herm_u[n][i] / X, w_i = rys_herm_w[n][i] / sqrt(X).",
              f"// Worst relative fit error per n: " + ", ".join(f"{worst[n]:.1e}" for n in range(1, NMAX + 1)),
              "#ifndef RYS_CONST", "#define RYS_CONST static const", "#endif",
              f"#define RYS_NMAX {NMAX}",
              f"#define RYS_DEG {DEG}",
              f"#define RYS_H {H!r}",
-             f"#define RYS_INV_H {1.0 / H!r}"]
-    nint = {n: int(round(xmax[n] / H)) for n in range(1, NMAX + 1)}
+             f"#define RYS_INV_H {1.0 / H!r}",
+             f"#define RYS_NSMALL {NSMALL}",
+             f"#define RYS_DEG_SMALL {DEG_SMALL}",
+             f"#define RYS_H_SMALL {H_SMALL!r}",
+             f"#define RYS_INV_H_SMALL {1.0 / H_SMALL!r}"]
+    nint = {n: int(round(xmax[n] / h_of(n))) for n in range(1, NMAX + 1)}
     off, acc = {}, 0
     for n in range(1, NMAX + 1):
         off[n] = acc
-        acc += nint[n] * 2 * n * (DEG + 1)
+        acc += nint[n] * 2 * n * (deg_of(n) + 1)
     lines.append(f"#define RYS_TABLE_SIZE {acc}")
     lines.append("RYS_CONST int rys_nint[RYS_NMAX + 1] = {0, " + ", ".join(str(nint[n]) for n in range(1, NMAX + 1)) + "};")
     lines.append("RYS_CONST int rys_offset[RYS_NMAX + 1] = {0, " + ", ".join(str(off[n]) for n in range(1, NMAX + 1)) + "};")

@@ -19,6 +19,11 @@ def _parse():
     text = INC.read_text()
     deg = int(re.search(r"#define RYS_DEG (\d+)", text).group(1))
     h = float(re.search(r"#define RYS_H ([0-9.]+)", text).group(1))
+    nsmall = int(re.search(r"#define RYS_NSMALL (\d+)", text).group(1))
+    deg_s = int(re.search(r"#define RYS_DEG_SMALL (\d+)", text).group(1))
+    h_s = float(re.search(r"#define RYS_H_SMALL ([0-9.]+)", text).group(1))
+    degs = [deg_s if n <= nsmall else deg for n in range(8)]
+    hs = [h_s if n <= nsmall else h for n in range(8)]
 
     def arr(name):
         body = re.search(rf"RYS_CONST \w+ {name}\[[^=]*=\s*\{{(.*?)\}};", text, re.S).group(1)
@@ -31,14 +36,15 @@ def _parse():
     hu = np.array(arr("rys_herm_u")[1:]).reshape(7, 7)  # leading {0} row collapses to one token
     hw = np.array(arr("rys_herm_w")[1:]).reshape(7, 7)
     table = np.array(arr("rys_table"))
-    return deg, h, nint, off, xmax, hu, hw, table
+    return degs, hs, nint, off, xmax, hu, hw, table
 
 
-DEG, H, NINT, OFF, XMAX, HU, HW, TABLE = _parse()
+DEGS, HS, NINT, OFF, XMAX, HU, HW, TABLE = _parse()
 
 
 def rule(n, X):
     if X < XMAX[n]:
+        DEG, H = DEGS[n], HS[n]
         j = int(X * (1.0 / H))
         s = X * (2.0 / H) - (2 * j + 1)
         base = OFF[n] + j * (2 * n * (DEG + 1))
@@ -54,9 +60,9 @@ def rule(n, X):
 
 
 def test_table_layout():
-    assert TABLE.shape[0] == sum(NINT[n] * 2 * n * (DEG + 1) for n in range(1, 8))
+    assert TABLE.shape[0] == sum(NINT[n] * 2 * n * (DEGS[n] + 1) for n in range(1, 8))
     for n in range(1, 8):
-        assert abs(NINT[n] * H - XMAX[n]) < 1e-12
+        assert abs(NINT[n] * HS[n] - XMAX[n]) < 1e-12
 
 
 @pytest.mark.parametrize("n", range(1, 8))
