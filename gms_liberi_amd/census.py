@@ -6,7 +6,7 @@ higher-L shell first in each pair, as the HIP kernels evaluate them.
 
 FLOP model (FMA = 2, add/sub/mul/div/sqrt = 1), per primitive quartet:
   setup 15 + Rys rule 2*nr*(2*deg) + per root [22 + 3 G tables + 3*ne*nf]
-(deg: the table's polynomial degree for nr roots, 7 for nr = 1, else 9)
+(deg: the table's polynomial degree for nr roots, 7 for nr <= 2, else 9)
 per contracted quartet:
   horizontal recurrence 2 per produced element + scale/cutoff 2*nabcd +
   digestion 12*nabcd (six block contractions).
@@ -33,7 +33,7 @@ CLASS_L = [(0, 0, 0, 0), (0, 0, 1, 0), (0, 0, 1, 1), (1, 0, 1, 1), (1, 0, 1, 0),
            (3, 3, 1, 0), (3, 3, 1, 1), (3, 3, 2, 0), (3, 3, 2, 1), (3, 3, 2, 2), (3, 3, 3, 2)]
 
 RYS_DEG = 9
-RYS_DEG_SMALL, RYS_NSMALL = 7, 1  # the one-root table
+RYS_DEG_SMALL, RYS_NSMALL = 7, 2  # the one- and two-root tables
 
 
 def rys_deg(nr: int) -> int:

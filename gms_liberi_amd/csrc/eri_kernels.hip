@@ -32,7 +32,7 @@ static const double rys_xmax_host[RYS_NMAX + 1] = {0.0, 39.0, 46.5, 53.5, 60.0, 
 namespace liberi {
 
 // polynomial degree, inverse interval width and intervals of the table of
-// the N-root rule (the one-root rule has its own, finer grid: RYS_NSMALL)
+// the N-root rule (the one- and two-root rules have their own, finer grid)
 __host__ __device__ constexpr int rys_deg(int n) { return n <= RYS_NSMALL ? RYS_DEG_SMALL : RYS_DEG; }
 __host__ __device__ constexpr double rys_inv_h(int n) { return n <= RYS_NSMALL ? RYS_INV_H_SMALL : RYS_INV_H; }
 __host__ __device__ constexpr int rys_ncoef(int n) { return 2 * n * (rys_deg(n) + 1); }

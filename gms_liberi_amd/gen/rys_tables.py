@@ -34,10 +34,10 @@ import numpy as np
 NMAX = 7
 H = 0.5          # interval width in X
 DEG = 9          # polynomial degree per interval
-# the one-root rule (the (ss|ss) and (ss|ps) classes, whose kernels read the
+# the one- and two-root rules (the s and p classes, whose kernels read the
 # table the most) on intervals of H_SMALL with degree DEG_SMALL: 20 % fewer
-# coefficients per evaluation for the same accuracy
-NSMALL = 1
+# coefficients per evaluation, fitted to 2e-15 relative or better
+NSMALL = 2
 H_SMALL = 0.25
 DEG_SMALL = 7
 DPS = 60
