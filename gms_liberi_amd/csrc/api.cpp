@@ -421,7 +421,7 @@ int liberi_c_get_stats(liberi_ctx *h, liberi_stats *out) {
     out->class_variant[c] = !cs_.active || cs_.variant < 0       ? -1
                             : h->last_fused && class_fused_nq(c) > 0 ? 4
                             : row                                 ? 1 + cs_.dev.transposed
-                            : cs_.variant == 3                    ? 3
+                            : cs_.variant == 3 || class_kernel_kind(c) == 2 ? 3
                                                                   : 0;
     out->class_fp32[c] = cs_.active && h->fp32_tau > 0.0 && cs_.fp32 == 1;
     out->class_tile[c] = cs_.active && row && cs_.tile >= 0 && !(h->last_fused && class_fused_nq(c) > 0)

@@ -198,7 +198,8 @@ typedef struct {
                                            launch with direct atomics, 1 row tiles over bra pairs,
                                            2 row tiles over ket pairs, 3 the cooperative workgroup-per-
                                            quartet form (group classes with an f shell, timed against
-                                           the group form at the first build), 4 inside the fused group
+                                           the group form at the first build; the only form of
+                                           (ff|dd), (ff|fd), (ff|ff)), 4 inside the fused group
                                            launch (LIBERI_OPT_FUSED); -1 class inactive or not yet built */
   int32_t class_fp32[LIBERI_NCLASS];    /* 1: the mixed form (LIBERI_OPT_FP32_TAU > 0) evaluates this
                                            class's qualifying quartets in FP32, chosen per class from
